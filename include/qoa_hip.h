@@ -1,0 +1,250 @@
+/*
+ * qoa_hip.h — C ABI of the MI355X (gfx950) state-vector backend for QOA's
+ * gate-application path.
+ *
+ * Every entry point is plain C: opaque handles, plain pointers and sizes.
+ * A state lives in HBM for its whole life; the host only touches it through
+ * qoa_state_upload / qoa_state_download / qoa_probabilities.
+ *
+ * Conventions
+ *   - Return value: 0 (QOA_OK) or a negative QOA_ERR_* code. Never aborts.
+ *     qoa_last_error() returns a human-readable message for the calling thread.
+ *   - Amplitudes are interleaved (re, im) doubles; basis index i has qubit q
+ *     in bit q (LSB = qubit 0), as in src/vectorization.rs:920-929.
+ *   - Each handle owns one HIP stream. Calls on a handle are stream-ordered;
+ *     calls that return host data synchronise that stream. Distinct handles
+ *     may be driven from distinct threads.
+ *   - Gate coefficients (cos/sin, 1/sqrt 2) are computed on the host with the
+ *     C library, like the reference (src/vectorization.rs:529-535, 650-651).
+ *
+ * Layer 1 (qoa_apply_*) mirrors the free functions of src/vectorization.rs
+ * one-for-one, including their parameter meaning (qubit index vs bit mask)
+ * and their out-of-range behaviour. Layer 2 (qoa_qs_*) mirrors the
+ * QuantumState wrappers of src/runtime/quantum_state.rs (bounds check,
+ * noise, renormalisation after every gate, measurement).
+ */
+#ifndef QOA_HIP_H
+#define QOA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define QOA_OK 0
+#define QOA_ERR_HANDLE (-1)     /* null or destroyed handle                  */
+#define QOA_ERR_QUBIT (-2)      /* qubit index out of range (layer 2)        */
+#define QOA_ERR_OOM (-3)        /* device allocation failed                  */
+#define QOA_ERR_HIP (-4)        /* a HIP runtime call failed                 */
+#define QOA_ERR_ARG (-5)        /* malformed argument                        */
+#define QOA_ERR_ADMISSION (-6)  /* request does not fit the memory budget    */
+#define QOA_ERR_UNSUPPORTED (-7)/* operation not available for this handle  */
+
+/* ---- precisions ------------------------------------------------------ */
+#define QOA_C128 0u  /* complex<double>, 16 B per amplitude (default)  */
+#define QOA_C64 1u   /* complex<float>, 8 B per amplitude (fused FP32)  */
+
+/* ---- gate op codes ---------------------------------------------------
+ * Where the reference bytecode has an opcode for the gate, the same byte is
+ * used (src/instructions.rs:631-830); ops without a bytecode use 0xE0+.    */
+enum qoa_op {
+  QOA_OP_H = 0x05,        /* q0                          (0x05 APPLYHADAMARD) */
+  QOA_OP_Z = 0x06,        /* q0                          (0x06 APPLYPHASEFLIP) */
+  QOA_OP_X = 0x07,        /* q0                          (0x07 APPLYBITFLIP) */
+  QOA_OP_PHASESHIFT = 0x08, /* q0, theta                 (0x08 PHASESHIFT)   */
+  QOA_OP_RESET = 0x0A,    /* q0                          (0x0A RESET)        */
+  QOA_OP_SWAP = 0x0B,     /* q0, q1                      (0x0B SWAP)         */
+  QOA_OP_CSWAP = 0x0C,    /* q0 = control, q1, q2        (0x0C CONTROLLEDSWAP) */
+  QOA_OP_T = 0x0D,        /* q0                          (0x0D APPLYTGATE)   */
+  QOA_OP_S = 0x0E,        /* q0                          (0x0E APPLYSGATE)   */
+  QOA_OP_RZ = 0x0F,       /* q0, theta                   (0x0F RZ)           */
+  QOA_OP_CNOT = 0x17,     /* q0 = control, q1 = target   (0x17 CNOT)         */
+  QOA_OP_RESET_ALL = 0x1C,/*                             (0x1C RESETALL)     */
+  QOA_OP_CZ = 0x1E,       /* q0 = control, q1 = target   (0x1E CZ)           */
+  QOA_OP_RX = 0x22,       /* q0, theta                   (0x22 RX)           */
+  QOA_OP_RY = 0x23,       /* q0, theta                   (0x23 RY)           */
+  QOA_OP_PHASE = 0x24,    /* q0, theta  == PHASESHIFT    (0x24 PHASE)        */
+  QOA_OP_MEASURE = 0x32,  /* q0                          (0x32 QMEAS)        */
+  QOA_OP_CPHASE = 0x35,   /* q0 = control, q1 = target, theta (0x35 CPHASE)  */
+  QOA_OP_Y = 0xE1,        /* q0  (reached in the reference only via noise)   */
+  QOA_OP_U2 = 0xE2,       /* q0; aux = index of a 2x2 matrix (8 doubles)     */
+  QOA_OP_U4 = 0xE3,       /* q0, q1; aux = index of a 4x4 matrix (32 doubles)*/
+  QOA_OP_TOFFOLI = 0xE4,  /* q0 = c1, q1 = c2, q2 = target                   */
+  QOA_OP_CY = 0xE5        /* q0 = control, q1 = target                       */
+};
+
+/* One gate of a circuit (32 bytes). */
+typedef struct qoa_gate {
+  uint32_t op;     /* enum qoa_op                         */
+  uint32_t q0, q1, q2;
+  double theta;    /* angle for parametrised gates        */
+  uint64_t aux;    /* matrix index for U2/U4, else 0      */
+} qoa_gate;
+
+/* Flags for circuit execution. */
+#define QOA_CZ_REF_MASKS 1u  /* CZ/CPHASE use the integer c|t as the bit mask,
+                                as QuantumState::apply_cz does
+                                (src/runtime/quantum_state.rs:338, 356-361).
+                                Default: textbook masks (1<<c)|(1<<t).       */
+
+/* ---- library --------------------------------------------------------- */
+int qoa_version(void);                      /* 10000*major + 100*minor + patch */
+const char* qoa_last_error(void);           /* thread-local, never NULL        */
+int qoa_device_info(char* name, size_t name_len, uint64_t* hbm_bytes, int* num_cus);
+
+/* ---- state lifecycle: QuantumState::new (quantum_state.rs:56-90) ------- */
+typedef struct qoa_state qoa_state;
+
+/* Allocates 2^max(n,1) amplitudes in HBM initialised to |0...0>. */
+int qoa_state_create(uint32_t n_qubits, uint32_t precision, qoa_state** out);
+int qoa_state_destroy(qoa_state* s);
+int qoa_state_num_qubits(const qoa_state* s, uint32_t* n_out);
+/* Host <-> device copies of the whole vector (interleaved re,im doubles;
+   for QOA_C64 handles the device converts to/from float). */
+int qoa_state_upload(qoa_state* s, const double* amps, uint64_t n_amps);
+int qoa_state_download(qoa_state* s, double* amps, uint64_t n_amps);
+int qoa_state_synchronize(qoa_state* s);
+/* Copies the amplitudes of src into dst (same n); c128 -> c64 and c64 ->
+   c128 conversions are allowed. Runs on dst's stream after src's work. */
+int qoa_state_copy(qoa_state* dst, const qoa_state* src);
+
+/* ---- layer 1: src/vectorization.rs free functions ---------------------- */
+/* apply_hadamard_vectorized(amps, norm_factor, mask_bit)   vectorization.rs:16   */
+int qoa_apply_hadamard(qoa_state* s, double nf_re, double nf_im, uint32_t mask_bit);
+/* apply_x_vectorized(amps, mask_bit)                       vectorization.rs:69   */
+int qoa_apply_x(qoa_state* s, uint32_t mask_bit);
+/* apply_y_vectorized(amps, mask_bit)                       vectorization.rs:118  */
+int qoa_apply_y(qoa_state* s, uint32_t mask_bit);
+/* apply_z_vectorized(amps, mask_bit)                       vectorization.rs:173  */
+int qoa_apply_z(qoa_state* s, uint32_t mask_bit);
+/* apply_t_vectorized(amps, mask_bit)                       vectorization.rs:217  */
+int qoa_apply_t(qoa_state* s, uint32_t mask_bit);
+/* apply_s_vectorized(amps, mask_bit)                       vectorization.rs:262  */
+int qoa_apply_s(qoa_state* s, uint32_t mask_bit);
+/* apply_phaseshift_vectorized(amps, mask_bit, angle)       vectorization.rs:307  */
+int qoa_apply_phaseshift(qoa_state* s, uint32_t mask_bit, double angle);
+/* apply_reset_vectorized(amps, mask_bit)                   vectorization.rs:352  */
+int qoa_apply_reset(qoa_state* s, uint32_t mask_bit);
+/* apply_swap_vectorized(amps, q1, q2)   (qubit indices)    vectorization.rs:421  */
+int qoa_apply_swap(qoa_state* s, uint32_t q1, uint32_t q2);
+/* apply_controlled_swap_vectorized(amps, c, t1, t2)        vectorization.rs:469  */
+int qoa_apply_controlled_swap(qoa_state* s, uint32_t control, uint32_t target1,
+                              uint32_t target2);
+/* apply_rx_vectorized(amps, mask_bit, angle)               vectorization.rs:528  */
+int qoa_apply_rx(qoa_state* s, uint32_t mask_bit, double angle);
+/* apply_ry_vectorized(amps, mask_bit, angle)               vectorization.rs:588  */
+int qoa_apply_ry(qoa_state* s, uint32_t mask_bit, double angle);
+/* apply_rz_vectorized(amps, mask_bit, angle)               vectorization.rs:648  */
+int qoa_apply_rz(qoa_state* s, uint32_t mask_bit, double angle);
+/* apply_cnot_vectorized(amps, control, target) (indices)   vectorization.rs:699  */
+int qoa_apply_cnot(qoa_state* s, uint32_t control, uint32_t target);
+/* apply_cz_vectorized(amps, control_mask, target_mask) (MASKS) vectorization.rs:745 */
+int qoa_apply_cz(qoa_state* s, uint64_t control_mask, uint64_t target_mask);
+/* apply_controlled_phase_rotation_vectorized(amps, cmask, tmask, angle)
+                                                           vectorization.rs:788  */
+int qoa_apply_controlled_phase_rotation(qoa_state* s, uint64_t control_mask,
+                                        uint64_t target_mask, double angle);
+/* apply_reset_all_vectorized(amps)                         vectorization.rs:857  */
+int qoa_apply_reset_all(qoa_state* s);
+
+/* General 2x2 on qubit q: m = {m00re,m00im,m01re,m01im,m10re,m10im,m11re,m11im},
+   computed as m00*a0 + m01*a1, m10*a0 + m11*a1 (apply_custom_unitary,
+   quantum_state.rs:1010-1062, without its noise/normalize step). */
+int qoa_apply_matrix2(qoa_state* s, uint32_t q, const double* m);
+/* General 4x4 on (q1,q2) with basis order (bit_high<<1)|bit_low, row-major
+   32 doubles (apply_two_qubit_unitary, quantum_state.rs:1064-1127). */
+int qoa_apply_matrix4(qoa_state* s, uint32_t q1, uint32_t q2, const double* m);
+/* Toffoli (quantum_state.rs:960-1008) and controlled-Y (:1163-1212) sweeps. */
+int qoa_apply_toffoli(qoa_state* s, uint32_t c1, uint32_t c2, uint32_t t);
+int qoa_apply_controlled_y(qoa_state* s, uint32_t c, uint32_t t);
+
+/* ---- reductions and collapse (quantum_state.rs:128-133, 399-458, 888-907) */
+/* sum_i |a_i|^2 (deterministic fixed-shape tree on the device). */
+int qoa_norm_sqr(qoa_state* s, double* out);
+/* normalize(): s = sum |a|^2; if s > 1e-12 divide every amplitude by sqrt(s),
+   else reset to |0...0>. Optional outputs (may be NULL). */
+int qoa_normalize(qoa_state* s, double* norm_sqr_out, int* was_reset);
+/* measure(q) with the uniform variate r supplied by the caller: p0/p1 sums,
+   outcome = (r < p0) ? 0 : 1, collapse with factor 1/sqrt(p) or 0 when
+   p <= 1e-12. Does NOT renormalise (the caller's wrapper does). */
+int qoa_measure(qoa_state* s, uint32_t q, double r, int* outcome, double* p0,
+                double* p1);
+/* |a_i|^2 for every i into out[0 .. n_amps). */
+int qoa_probabilities(qoa_state* s, double* out, uint64_t n_amps);
+/* One depolarizing event set: for every qubit q with code[q] in {1,2,3}
+   apply X, Y or Z on q (0 = none), in qubit order, as one fused sweep
+   (apply_depolarizing_noise, quantum_state.rs:872-885). */
+int qoa_apply_pauli_codes(qoa_state* s, const uint8_t* codes, uint32_t n_codes);
+
+/* ---- circuits ---------------------------------------------------------- */
+/* Runs gates in order with one sweep kernel per gate (layer-1 semantics: no
+   renormalisation, no noise). MEASURE/RESET are not allowed here. */
+int qoa_run_gates(qoa_state* s, const qoa_gate* gates, uint32_t n_gates,
+                  const double* matrices, uint32_t flags);
+
+/* Fused execution: the gate list is planned into passes; each pass loads
+   2^tile_qubits-amplitude tiles (the low `low_qubits` qubits plus qubits
+   chosen per pass) into LDS once, applies every gate of the pass there and
+   writes the tile back. tile_qubits = 0 picks the default. Results equal
+   the per-gate run of the same gates in planned order (bit for bit). */
+typedef struct qoa_fuse_opts {
+  uint32_t tile_qubits;  /* k: 2^k amplitudes per tile (0 = default)     */
+  uint32_t low_qubits;   /* m: qubits 0..m-1 always in the tile (0 = default) */
+  uint32_t flags;        /* QOA_CZ_REF_MASKS                            */
+  uint32_t reserved;
+} qoa_fuse_opts;
+int qoa_run_fused(qoa_state* s, const qoa_gate* gates, uint32_t n_gates,
+                  const double* matrices, const qoa_fuse_opts* opts);
+/* Planner only (host): writes the executed order of gate indices into
+   order[0..n_gates) and the pass boundary count into *n_passes. */
+int qoa_plan_fused(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
+                   const qoa_fuse_opts* opts, uint32_t* order, uint32_t* n_passes);
+
+/* Kernel timing: when enabled, every kernel launched on the handle's stream
+   is bracketed by HIP events; qoa_timing_read returns per-kind totals. */
+int qoa_timing_enable(qoa_state* s, int enable);
+/* kind names: "sweep_dense", "sweep_diag", "sweep_perm", "fused_pass",
+   "reduce", "other". Returns total ms and launch count for that kind. */
+int qoa_timing_read(qoa_state* s, const char* kind, double* total_ms,
+                    uint64_t* launches, double* bytes);
+
+/* ---- layer 2: QuantumState wrappers (src/runtime/quantum_state.rs) ------ */
+#define QOA_NOISE_NONE 0   /* noise_config: None           */
+#define QOA_NOISE_IDEAL 1  /* Some(NoiseConfig::Ideal)     */
+#define QOA_NOISE_RANDOM 2 /* Some(NoiseConfig::Random)    */
+#define QOA_NOISE_FIXED 3  /* Some(NoiseConfig::Fixed(p))  */
+int qoa_qs_set_noise(qoa_state* s, int kind, double p);
+/* seed_rng (quantum_state.rs:956-958): the handle's Philox4x32-10 stream. */
+int qoa_qs_seed_rng(qoa_state* s, uint64_t seed);
+int qoa_qs_set_flags(qoa_state* s, uint32_t flags);  /* QOA_CZ_REF_MASKS */
+/* Applies one gate as QuantumState::apply_* does: bounds check (error ->
+   QOA_ERR_QUBIT, state untouched), the sweep, apply_noise, normalize. */
+int qoa_qs_apply(qoa_state* s, const qoa_gate* g, const double* matrices);
+/* QuantumState::measure (quantum_state.rs:399-458): draws r from the
+   handle's stream, collapses, then apply_noise + normalize. */
+int qoa_qs_measure(qoa_state* s, uint32_t q, int* outcome);
+/* measure_all (quantum_state.rs:909-918): measures qubits 0..n-1 in order on
+   a copy; the handle's state is left unchanged. out has n entries. */
+int qoa_qs_measure_all(qoa_state* s, uint8_t* out, uint32_t n_out);
+/* apply_noise + normalize as run after every gate (quantum_state.rs:839-907). */
+int qoa_qs_apply_noise(qoa_state* s);
+
+/* ---- trajectories and batches (many states, one launch per layer) ------- */
+typedef struct qoa_batch qoa_batch;
+typedef struct qoa_job_desc {
+  uint32_t n_qubits;     /* 1..30                                         */
+  uint32_t n_gates;      /* gates of this job in the shared gate array     */
+  uint64_t gate_offset;  /* first gate index in the shared gate array      */
+  uint64_t noise_seed;   /* Philox key for the job's noise trajectories    */
+  uint32_t trajectories; /* >= 1; each is an independent state             */
+  uint32_t precision;    /* QOA_C128 or QOA_C64                            */
+  double p1, p2;         /* Pauli error prob after 1q / 2q gates (0 = none)*/
+} qoa_job_desc;
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QOA_HIP_H */

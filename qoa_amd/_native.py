@@ -1,0 +1,161 @@
+"""ctypes binding of libqoa_hip.so (the C ABI declared in include/qoa_hip.h).
+
+The library is built in-tree by ``python -m qoa_amd.build``. Loading fails
+loudly when it is missing: there is no CPU fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libqoa_hip.so"
+HEADER = PKG.parent / "include" / "qoa_hip.h"
+
+QOA_OK = 0
+QOA_ERR_HANDLE = -1
+QOA_ERR_QUBIT = -2
+QOA_ERR_OOM = -3
+QOA_ERR_HIP = -4
+QOA_ERR_ARG = -5
+QOA_ERR_ADMISSION = -6
+QOA_ERR_UNSUPPORTED = -7
+
+QOA_C128 = 0
+QOA_C64 = 1
+QOA_CZ_REF_MASKS = 1
+
+NOISE_NONE, NOISE_IDEAL, NOISE_RANDOM, NOISE_FIXED = 0, 1, 2, 3
+
+# op codes (reference bytecode bytes where they exist, src/instructions.rs:631-830)
+OP = dict(H=0x05, Z=0x06, X=0x07, PHASESHIFT=0x08, RESET=0x0A, SWAP=0x0B, CSWAP=0x0C, T=0x0D,
+          S=0x0E, RZ=0x0F, CNOT=0x17, RESET_ALL=0x1C, CZ=0x1E, RX=0x22, RY=0x23, PHASE=0x24,
+          MEASURE=0x32, CPHASE=0x35, Y=0xE1, U2=0xE2, U4=0xE3, TOFFOLI=0xE4, CY=0xE5)
+
+GATE_DTYPE = np.dtype([("op", "<u4"), ("q0", "<u4"), ("q1", "<u4"), ("q2", "<u4"),
+                       ("theta", "<f8"), ("aux", "<u8")], align=True)
+assert GATE_DTYPE.itemsize == 32
+
+
+class QoaGate(C.Structure):
+    _fields_ = [("op", C.c_uint32), ("q0", C.c_uint32), ("q1", C.c_uint32), ("q2", C.c_uint32),
+                ("theta", C.c_double), ("aux", C.c_uint64)]
+
+
+class QoaFuseOpts(C.Structure):
+    _fields_ = [("tile_qubits", C.c_uint32), ("low_qubits", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class QoaJobDesc(C.Structure):
+    _fields_ = [("n_qubits", C.c_uint32), ("n_gates", C.c_uint32), ("gate_offset", C.c_uint64),
+                ("noise_seed", C.c_uint64), ("trajectories", C.c_uint32),
+                ("precision", C.c_uint32), ("p1", C.c_double), ("p2", C.c_double)]
+
+
+class QoaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+P = C.c_void_p
+D = C.c_double
+U32 = C.c_uint32
+U64 = C.c_uint64
+I32 = C.c_int
+DP = C.POINTER(C.c_double)
+
+_SIGS = {
+    "qoa_version": ([], I32),
+    "qoa_last_error": ([], C.c_char_p),
+    "qoa_device_info": ([C.c_char_p, C.c_size_t, C.POINTER(U64), C.POINTER(I32)], I32),
+    "qoa_state_create": ([U32, U32, C.POINTER(P)], I32),
+    "qoa_state_destroy": ([P], I32),
+    "qoa_state_num_qubits": ([P, C.POINTER(U32)], I32),
+    "qoa_state_upload": ([P, P, U64], I32),
+    "qoa_state_download": ([P, P, U64], I32),
+    "qoa_state_synchronize": ([P], I32),
+    "qoa_state_copy": ([P, P], I32),
+    "qoa_apply_hadamard": ([P, D, D, U32], I32),
+    "qoa_apply_x": ([P, U32], I32),
+    "qoa_apply_y": ([P, U32], I32),
+    "qoa_apply_z": ([P, U32], I32),
+    "qoa_apply_t": ([P, U32], I32),
+    "qoa_apply_s": ([P, U32], I32),
+    "qoa_apply_phaseshift": ([P, U32, D], I32),
+    "qoa_apply_reset": ([P, U32], I32),
+    "qoa_apply_swap": ([P, U32, U32], I32),
+    "qoa_apply_controlled_swap": ([P, U32, U32, U32], I32),
+    "qoa_apply_rx": ([P, U32, D], I32),
+    "qoa_apply_ry": ([P, U32, D], I32),
+    "qoa_apply_rz": ([P, U32, D], I32),
+    "qoa_apply_cnot": ([P, U32, U32], I32),
+    "qoa_apply_cz": ([P, U64, U64], I32),
+    "qoa_apply_controlled_phase_rotation": ([P, U64, U64, D], I32),
+    "qoa_apply_reset_all": ([P], I32),
+    "qoa_apply_matrix2": ([P, U32, P], I32),
+    "qoa_apply_matrix4": ([P, U32, U32, P], I32),
+    "qoa_apply_toffoli": ([P, U32, U32, U32], I32),
+    "qoa_apply_controlled_y": ([P, U32, U32], I32),
+    "qoa_norm_sqr": ([P, DP], I32),
+    "qoa_normalize": ([P, DP, C.POINTER(I32)], I32),
+    "qoa_measure": ([P, U32, D, C.POINTER(I32), DP, DP], I32),
+    "qoa_probabilities": ([P, P, U64], I32),
+    "qoa_apply_pauli_codes": ([P, P, U32], I32),
+    "qoa_run_gates": ([P, P, U32, P, U32], I32),
+    "qoa_run_fused": ([P, P, U32, P, P], I32),
+    "qoa_plan_fused": ([U32, P, U32, P, P, C.POINTER(U32)], I32),
+    "qoa_timing_enable": ([P, I32], I32),
+    "qoa_timing_read": ([P, C.c_char_p, DP, C.POINTER(U64), DP], I32),
+    "qoa_qs_set_noise": ([P, I32, D], I32),
+    "qoa_qs_seed_rng": ([P, U64], I32),
+    "qoa_qs_set_flags": ([P, U32], I32),
+    "qoa_qs_apply": ([P, P, P], I32),
+    "qoa_qs_measure": ([P, U32, C.POINTER(I32)], I32),
+    "qoa_qs_measure_all": ([P, P, U32], I32),
+    "qoa_qs_apply_noise": ([P], I32),
+}
+
+_lib = None
+
+
+def header_functions() -> list[str]:
+    """Names of every function declared in include/qoa_hip.h."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(qoa_\w+)\s*\(", text, flags=re.M)))
+
+
+def load(path: Path | None = None) -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it with `python -m qoa_amd.build` "
+                           "(the HIP backend has no CPU fallback)")
+    lib = C.CDLL(str(p))
+    for name, (args, res) in _SIGS.items():
+        f = getattr(lib, name, None)
+        if f is None:
+            continue
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        msg = load().qoa_last_error().decode(errors="replace")
+        raise QoaError(rc, msg)
+    return rc
+
+
+def ptr(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)

