@@ -94,6 +94,10 @@ typedef struct qoa_gate {
 int qoa_version(void);                      /* 10000*major + 100*minor + patch */
 const char* qoa_last_error(void);           /* thread-local, never NULL        */
 int qoa_device_info(char* name, size_t name_len, uint64_t* hbm_bytes, int* num_cus);
+/* Library options. "index64" (0/1): use 64-bit index arithmetic in every
+   kernel even for states of at most 31 qubits (the path states of 32+ qubits
+   take), so that it can be tested on small states. */
+int qoa_set_option(const char* key, int64_t value);
 
 /* ---- state lifecycle: QuantumState::new (quantum_state.rs:56-90) ------- */
 typedef struct qoa_state qoa_state;

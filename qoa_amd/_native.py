@@ -73,6 +73,7 @@ _SIGS = {
     "qoa_version": ([], I32),
     "qoa_last_error": ([], C.c_char_p),
     "qoa_device_info": ([C.c_char_p, C.c_size_t, C.POINTER(U64), C.POINTER(I32)], I32),
+    "qoa_set_option": ([C.c_char_p, C.c_int64], I32),
     "qoa_state_create": ([U32, U32, C.POINTER(P)], I32),
     "qoa_state_destroy": ([P], I32),
     "qoa_state_num_qubits": ([P, C.POINTER(U32)], I32),

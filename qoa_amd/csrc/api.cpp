@@ -28,6 +28,8 @@ int set_error(int code, const char* fmt, ...) {
 
 bool valid(const qoa_state* s) { return s != nullptr && s->magic == kMagic; }
 
+bool g_force_index64 = false;
+
 const char* kind_name(int k) {
   static const char* names[KK_COUNT] = {"sweep_dense", "sweep_diag", "sweep_perm",
                                         "fused_pass",  "reduce",     "other"};
@@ -216,6 +218,15 @@ int qoa_device_info(char* name, size_t name_len, uint64_t* hbm_bytes, int* num_c
   return QOA_OK;
 }
 
+int qoa_set_option(const char* key, int64_t value) {
+  if (!key) return set_error(QOA_ERR_ARG, "key is NULL");
+  if (std::strcmp(key, "index64") == 0) {
+    g_force_index64 = value != 0;
+    return QOA_OK;
+  }
+  return set_error(QOA_ERR_ARG, "unknown option '%s'", key);
+}
+
 // ---- lifecycle -------------------------------------------------------------------
 int qoa_state_create(uint32_t n_qubits, uint32_t precision, qoa_state** out) {
   if (!out) return set_error(QOA_ERR_ARG, "out is NULL");
@@ -231,6 +242,7 @@ int qoa_state_create(uint32_t n_qubits, uint32_t precision, qoa_state** out) {
   s->precision = precision;
   s->amps = nullptr;
   s->red = nullptr;
+  s->fused_cache = nullptr;
   s->stream = nullptr;
   s->noise_kind = QOA_NOISE_NONE;
   s->noise_p = 0.0;
@@ -301,6 +313,7 @@ int qoa_state_destroy(qoa_state* s) {
     hipEventDestroy(t.stop);
   }
   for (auto e : s->event_pool) hipEventDestroy(e);
+  fused_cache_free(s);
   if (s->amps) hipFree(s->amps);
   if (s->red) hipFree(s->red);
   if (s->stream) hipStreamDestroy(s->stream);

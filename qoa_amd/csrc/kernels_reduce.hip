@@ -216,7 +216,7 @@ int launch_norm_sum(qoa_state* s, uint64_t selbit, bool two_way, int slot) {
   const int nb = (int)(nb64 < (uint64_t)kRedBlocks ? (nb64 ? nb64 : 1) : kRedBlocks);
   double* part = s->red;
   timing_begin(s, KK_REDUCE, 16.0 * (double)s->len);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_norm_partials<uint32_t>), dim3(nb), dim3(kBlock), 0, s->stream, a,
                        (uint32_t)s->len, (uint32_t)selbit, part);
   else
@@ -232,7 +232,7 @@ int launch_norm_sum(qoa_state* s, uint64_t selbit, bool two_way, int slot) {
 int launch_normalize_apply(qoa_state* s, int slot) {
   double2* a = (double2*)s->amps;
   timing_begin(s, KK_REDUCE, 32.0 * (double)s->len);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_normalize_apply<uint32_t>), dim3(grid4(s->len)), dim3(kBlock), 0,
                        s->stream, a, (uint32_t)s->len, s->red + kSlotBase + slot);
   else
@@ -246,7 +246,7 @@ int launch_normalize_apply(qoa_state* s, int slot) {
 int launch_collapse(qoa_state* s, uint64_t bit, int outcome, double f) {
   double2* a = (double2*)s->amps;
   timing_begin(s, KK_REDUCE, 24.0 * (double)s->len);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_collapse<uint32_t>), dim3(grid4(s->len)), dim3(kBlock), 0, s->stream, a,
                        (uint32_t)s->len, (uint32_t)bit, outcome, f);
   else
@@ -260,7 +260,7 @@ int launch_collapse(qoa_state* s, uint64_t bit, int outcome, double f) {
 int launch_reset_apply(qoa_state* s, uint64_t bit, int slot) {
   double2* a = (double2*)s->amps;
   timing_begin(s, KK_REDUCE, 24.0 * (double)s->len);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_reset_apply<uint32_t>), dim3(grid4(s->len)), dim3(kBlock), 0, s->stream,
                        a, (uint32_t)s->len, (uint32_t)bit, s->red + kSlotBase + slot);
   else
@@ -274,7 +274,7 @@ int launch_reset_apply(qoa_state* s, uint64_t bit, int slot) {
 int launch_probabilities(qoa_state* s, double* out) {
   const double2* a = (const double2*)s->amps;
   timing_begin(s, KK_REDUCE, 24.0 * (double)s->len);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_probs<uint32_t>), dim3(grid4(s->len)), dim3(kBlock), 0, s->stream, a,
                        (uint32_t)s->len, out);
   else

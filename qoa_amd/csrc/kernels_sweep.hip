@@ -319,7 +319,7 @@ int launch_2x2(qoa_state* s, uint32_t t, uint64_t cmask, Op op, int kind) {
   timing_begin(s, kind, sweep_bytes(s, cmask));
   if (tv < 6) {
     Ins ins = make_ins(cmask);
-    if (s->n <= 31)
+    if (use32(s))
       hipLaunchKernelGGL((k_shfl<uint32_t, Op>), dim3(grid_for(nelem)), dim3(kBlock), 0, s->stream,
                          a, (uint32_t)nelem, ins, (uint32_t)cmask, tv, op);
     else
@@ -328,7 +328,7 @@ int launch_2x2(qoa_state* s, uint32_t t, uint64_t cmask, Op op, int kind) {
   } else {
     Ins ins = make_ins(cmask | (1ULL << t));
     const uint64_t np = nelem >> 1;
-    if (s->n <= 31)
+    if (use32(s))
       hipLaunchKernelGGL((k_pair<uint32_t, Op>), dim3(grid_for(np)), dim3(kBlock), 0, s->stream, a,
                          (uint32_t)np, ins, (uint32_t)(1u << t), (uint32_t)cmask, op);
     else
@@ -360,7 +360,7 @@ int launch_diag_mask(qoa_state* s, uint64_t mask, bool neg, Cx f) {
   const uint64_t count = s->len >> popc(mask);
   Ins ins = make_ins(mask);
   timing_begin(s, KK_DIAG, sweep_bytes(s, mask) * (mask ? 1.0 : 1.0));
-  if (s->n <= 31) {
+  if (use32(s)) {
     if (neg)
       hipLaunchKernelGGL((k_diag<uint32_t, true>), dim3(grid_for(count)), dim3(kBlock), 0,
                          s->stream, a, (uint32_t)count, ins, (uint32_t)mask, d2(f));
@@ -383,7 +383,7 @@ int launch_diag_mask(qoa_state* s, uint64_t mask, bool neg, Cx f) {
 int launch_rz(qoa_state* s, uint64_t bit, Cx m00, Cx m11) {
   double2* a = (double2*)s->amps;
   timing_begin(s, KK_DIAG, 32.0 * (double)s->len);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_rz<uint32_t>), dim3(grid_for(s->len)), dim3(kBlock), 0, s->stream, a,
                        (uint32_t)s->len, (uint32_t)bit, d2(m00), d2(m11));
   else
@@ -405,7 +405,7 @@ int launch_swap2(qoa_state* s, uint32_t q1, uint32_t q2, uint64_t cmask) {
   double bytes = 32.0 * (double)s->len / (double)(1ULL << popc(cmask & ~3ULL));
   if (q1 >= 2 && q2 >= 2) bytes *= 0.5;
   timing_begin(s, KK_PERM, bytes);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_swap2<uint32_t>), dim3(grid_for(count)), dim3(kBlock), 0, s->stream, a,
                        (uint32_t)count, ins, (uint32_t)cmask, (uint32_t)m1, (uint32_t)m2);
   else
@@ -424,7 +424,7 @@ int launch_matrix4(qoa_state* s, uint32_t q1, uint32_t q2, const double* m32) {
   Ins ins = make_ins((1ULL << lo) | (1ULL << hi));
   const uint64_t count = s->len >> 2;
   timing_begin(s, KK_DENSE, 32.0 * (double)s->len);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_mat4<uint32_t>), dim3(grid_for(count)), dim3(kBlock), 0, s->stream, a,
                        (uint32_t)count, ins, (uint32_t)(1u << lo), (uint32_t)(1u << hi), u);
   else
@@ -451,14 +451,14 @@ int launch_pauli_codes(qoa_state* s, const uint8_t* codes, uint32_t n_codes) {
   if (xmask) {
     const int low = __builtin_ctzll(xmask);
     const uint64_t np = s->len >> 1;
-    if (s->n <= 31)
+    if (use32(s))
       hipLaunchKernelGGL((k_pauli_pairs<uint32_t>), dim3(grid_for(np)), dim3(kBlock), 0, s->stream,
                          a, (uint32_t)np, low, (uint32_t)xmask, pl);
     else
       hipLaunchKernelGGL((k_pauli_pairs<uint64_t>), dim3(grid_for(np)), dim3(kBlock), 0, s->stream,
                          a, (uint64_t)np, low, xmask, pl);
   } else {
-    if (s->n <= 31)
+    if (use32(s))
       hipLaunchKernelGGL((k_pauli_diag<uint32_t>), dim3(grid_for(s->len)), dim3(kBlock), 0,
                          s->stream, a, (uint32_t)s->len, pl);
     else
@@ -473,7 +473,7 @@ int launch_pauli_codes(qoa_state* s, const uint8_t* codes, uint32_t n_codes) {
 int launch_set_basis0(qoa_state* s) {
   double2* a = (double2*)s->amps;
   timing_begin(s, KK_OTHER, 16.0 * (double)s->len);
-  if (s->n <= 31)
+  if (use32(s))
     hipLaunchKernelGGL((k_set_basis0<uint32_t>), dim3(grid_for(s->len)), dim3(kBlock), 0,
                        s->stream, a, (uint32_t)s->len);
   else

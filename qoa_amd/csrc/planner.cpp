@@ -1,0 +1,370 @@
+// planner.cpp — splits a gate list into LDS-tile passes.
+//
+// A pass owns a set S of k qubits (always the low m qubits, for coalesced
+// HBM runs of 2^m amplitudes, plus k-m qubits chosen per pass). Each
+// workgroup loads the 2^k amplitudes that share one assignment of the other
+// n-k bits, applies the pass's gates in LDS and writes them back, so every
+// pass costs one read and one write of the state.
+//
+// A gate can run in a pass when every qubit it mixes (targets of non-diagonal
+// actions) is in S and none of its qubits was blocked by an earlier gate left
+// for a later pass. Qubits a gate only reads (controls, diagonal phases) may
+// lie outside S: their bit is constant inside a tile, so the gate becomes a
+// per-tile condition (CNOT with an outside control is "X or nothing"; a phase
+// on an outside qubit is a tile-wide scalar). Gates keep their original
+// relative order inside a pass; gates moved across passes act on disjoint
+// qubits, so the planned order is a valid reordering of the circuit.
+#include "planner.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+#include "qoa_internal.hpp"
+
+namespace qoa {
+namespace {
+
+enum FK { FK_H, FK_M, FK_X, FK_Y, FK_DNEG, FK_DPHASE, FK_RZ, FK_SWAP, FK_MAT4 };
+
+struct FOp {
+  int kind = 0;
+  int t = -1, t2 = -1;  // mixed (active) qubits
+  uint64_t pmask = 0;   // read-only qubits that must be 1
+  int rzq = -1;         // RZ: read-only qubit whose bit picks the factor
+  double c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double* m4 = nullptr;
+  uint32_t gi = 0;
+  uint64_t active = 0, all = 0;
+};
+
+inline void polar(double th, double* out) {  // Complex64::from_polar(1.0, th)
+  out[0] = 1.0 * std::cos(th);
+  out[1] = 1.0 * std::sin(th);
+}
+
+inline uint64_t bit(int q) { return 1ULL << q; }
+
+// Turns one gate into at most one fused op with the layer-1 semantics of the
+// per-gate entry points (out-of-range qubits, degenerate controls).
+int canonical(uint32_t n, const qoa_gate& g, const double* mats, uint32_t flags,
+              std::vector<FOp>* out, uint32_t gi) {
+  FOp f;
+  f.gi = gi;
+  auto in = [n](uint32_t q) { return q < n; };
+  const uint32_t q0 = g.q0, q1 = g.q1, q2 = g.q2;
+  switch (g.op) {
+    case QOA_OP_H:
+      if (!in(q0)) return 0;
+      f.kind = FK_H; f.t = q0; f.c[0] = 1.0 / M_SQRT2; f.c[1] = 0.0;
+      break;
+    case QOA_OP_X:
+      if (!in(q0)) return 0;
+      f.kind = FK_X; f.t = q0;
+      break;
+    case QOA_OP_Y:
+      if (!in(q0)) return 0;
+      f.kind = FK_Y; f.t = q0;
+      break;
+    case QOA_OP_Z:
+      if (!in(q0)) return 0;
+      f.kind = FK_DNEG; f.pmask = bit(q0);
+      break;
+    case QOA_OP_S: case QOA_OP_T: case QOA_OP_PHASESHIFT: case QOA_OP_PHASE:
+      if (!in(q0)) return 0;
+      f.kind = FK_DPHASE; f.pmask = bit(q0);
+      polar(g.op == QOA_OP_S ? M_PI / 2.0 : g.op == QOA_OP_T ? M_PI / 4.0 : g.theta, f.c);
+      break;
+    case QOA_OP_RX: case QOA_OP_RY: {
+      if (!in(q0)) return 0;
+      const double c = std::cos(g.theta / 2.0), s = std::sin(g.theta / 2.0);
+      f.kind = FK_M; f.t = q0;
+      if (g.op == QOA_OP_RX) {
+        const double m[8] = {c, 0.0, 0.0, -s, 0.0, -s, c, 0.0};
+        std::copy(m, m + 8, f.c);
+      } else {
+        const double m[8] = {c, 0.0, -s, 0.0, s, 0.0, c, 0.0};
+        std::copy(m, m + 8, f.c);
+      }
+      break;
+    }
+    case QOA_OP_RZ:
+      polar(-g.theta / 2.0, f.c);
+      polar(g.theta / 2.0, f.c + 2);
+      if (in(q0)) {
+        f.kind = FK_RZ; f.rzq = q0;
+      } else {  // every index has the bit clear: global factor m00
+        f.kind = FK_DPHASE; f.pmask = 0;
+      }
+      break;
+    case QOA_OP_U2:
+      if (!mats) return set_error(QOA_ERR_ARG, "U2 gate without a matrix table");
+      if (!in(q0)) return 0;
+      f.kind = FK_M; f.t = q0;
+      std::copy(mats + 8 * g.aux, mats + 8 * g.aux + 8, f.c);
+      break;
+    case QOA_OP_CNOT:
+      if (!in(q0) || !in(q1) || q0 == q1) return 0;
+      f.kind = FK_X; f.t = q1; f.pmask = bit(q0);
+      break;
+    case QOA_OP_CY:
+      if (!in(q0) || !in(q1)) return set_error(QOA_ERR_QUBIT, "controlled-y qubit out of bounds");
+      if (q0 == q1) return set_error(QOA_ERR_ARG, "control and target qubits cannot be the same");
+      f.kind = FK_Y; f.t = q1; f.pmask = bit(q0);
+      break;
+    case QOA_OP_TOFFOLI:
+      if (!in(q0) || !in(q1) || !in(q2)) return set_error(QOA_ERR_QUBIT, "toffoli qubit out of bounds");
+      if (q0 == q1 || q0 == q2 || q1 == q2)
+        return set_error(QOA_ERR_ARG, "control and target qubits must be distinct for toffoli gate");
+      f.kind = FK_X; f.t = q2; f.pmask = bit(q0) | bit(q1);
+      break;
+    case QOA_OP_CZ: case QOA_OP_CPHASE: {
+      uint64_t m;
+      if (flags & QOA_CZ_REF_MASKS) {
+        m = (uint64_t)q0 | (uint64_t)q1;
+      } else {
+        if (q0 >= 64 || q1 >= 64) return set_error(QOA_ERR_ARG, "qubit index >= 64");
+        m = bit(q0) | bit(q1);
+      }
+      if (n < 64 && (m >> n)) return 0;  // (i & m) == m never holds
+      f.pmask = m;
+      if (g.op == QOA_OP_CZ) f.kind = FK_DNEG;
+      else { f.kind = FK_DPHASE; polar(g.theta, f.c); }
+      break;
+    }
+    case QOA_OP_SWAP:
+      if (!in(q0) || !in(q1) || q0 == q1) return 0;
+      f.kind = FK_SWAP; f.t = q0; f.t2 = q1;
+      break;
+    case QOA_OP_CSWAP:
+      if (!in(q0) || !in(q1) || !in(q2) || q1 == q2) return 0;
+      if (q0 == q1) { f.kind = FK_X; f.t = q2; f.pmask = bit(q0); }
+      else if (q0 == q2) { f.kind = FK_X; f.t = q1; f.pmask = bit(q0); }
+      else { f.kind = FK_SWAP; f.t = q1; f.t2 = q2; f.pmask = bit(q0); }
+      break;
+    case QOA_OP_U4:
+      if (!mats) return set_error(QOA_ERR_ARG, "U4 gate without a matrix table");
+      if (!in(q0) || !in(q1)) return set_error(QOA_ERR_QUBIT, "two-qubit unitary qubit out of bounds");
+      if (q0 == q1) return set_error(QOA_ERR_ARG, "qubits cannot be the same for two-qubit unitary");
+      f.kind = FK_MAT4; f.t = q0; f.t2 = q1; f.m4 = mats + 32 * g.aux;
+      break;
+    default:
+      return set_error(QOA_ERR_ARG, "op 0x%02x cannot run in a fused circuit", g.op);
+  }
+  f.active = (f.t >= 0 ? bit(f.t) : 0) | (f.t2 >= 0 ? bit(f.t2) : 0);
+  f.all = f.active | f.pmask | (f.rzq >= 0 ? bit(f.rzq) : 0);
+  out->push_back(f);
+  return 0;
+}
+
+// Number of ops of `rem` that a pass over S would run.
+int simulate(const std::vector<FOp>& ops, const std::vector<uint32_t>& rem, uint64_t S,
+             uint64_t allq) {
+  uint64_t blocked = 0;
+  int cnt = 0;
+  for (uint32_t i : rem) {
+    const FOp& f = ops[i];
+    if ((f.all & blocked) == 0 && (f.active & ~S) == 0) {
+      ++cnt;
+    } else {
+      blocked |= f.all;
+      if ((blocked & allq) == allq) break;
+    }
+  }
+  return cnt;
+}
+
+inline int loc_of(const uint8_t* spos, int k, int q) {
+  for (int j = 0; j < k; ++j)
+    if (spos[j] == q) return j;
+  return -1;
+}
+
+uint32_t local_bits(uint64_t mask, const uint8_t* spos, int k) {
+  uint32_t r = 0;
+  for (int j = 0; j < k; ++j)
+    if (mask & bit(spos[j])) r |= 1u << j;
+  return r;
+}
+
+void set_ins(TileOp* op, uint32_t bits) {
+  int cnt = __builtin_popcount(bits);
+  if (cnt > 4) {
+    op->nins = 0xFF;
+    return;
+  }
+  op->nins = 0;
+  for (int j = 0; j < 32; ++j)
+    if (bits & (1u << j)) op->ins[op->nins++] = (uint8_t)j;
+}
+
+void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, FusedPlan* plan) {
+  TileOp op{};
+  const uint64_t pin = f.pmask & S, pout = f.pmask & ~S;
+  op.lmask = local_bits(pin, spos, k);
+  op.gmask = pout;
+  op.gval = pout;
+  std::copy(f.c, f.c + 8, op.c);
+  switch (f.kind) {
+    case FK_H: case FK_M: case FK_X: case FK_Y:
+      op.kind = f.kind == FK_H ? TK_H : f.kind == FK_M ? TK_M : f.kind == FK_X ? TK_X : TK_Y;
+      op.t = (uint8_t)loc_of(spos, k, f.t);
+      set_ins(&op, op.lmask | (1u << op.t));
+      break;
+    case FK_DNEG: case FK_DPHASE:
+      op.kind = f.kind == FK_DNEG ? TK_DNEG : TK_DPHASE;
+      set_ins(&op, op.lmask);
+      break;
+    case FK_RZ:
+      if (S & bit(f.rzq)) {
+        op.kind = TK_RZ;
+        op.t = (uint8_t)loc_of(spos, k, f.rzq);
+        op.nins = 0;
+      } else {  // bit constant per tile: two complementary tile-wide factors
+        TileOp o0 = op, o1 = op;
+        o0.kind = o1.kind = TK_DPHASE;
+        o0.lmask = o1.lmask = 0;
+        o0.nins = o1.nins = 0;
+        o0.gmask = o1.gmask = bit(f.rzq);
+        o0.gval = 0;
+        o1.gval = bit(f.rzq);
+        o1.c[0] = f.c[2];
+        o1.c[1] = f.c[3];
+        plan->ops.push_back(o0);
+        plan->ops.push_back(o1);
+        return;
+      }
+      break;
+    case FK_SWAP:
+      op.kind = TK_SWAP;
+      op.t = (uint8_t)loc_of(spos, k, f.t);
+      op.t2 = (uint8_t)loc_of(spos, k, f.t2);
+      set_ins(&op, op.lmask | (1u << op.t) | (1u << op.t2));
+      break;
+    case FK_MAT4:
+      op.kind = TK_MAT4;
+      op.t = (uint8_t)loc_of(spos, k, f.t);
+      op.t2 = (uint8_t)loc_of(spos, k, f.t2);
+      op.aux = (uint32_t)plan->pool.size();
+      plan->pool.insert(plan->pool.end(), f.m4, f.m4 + 32);
+      set_ins(&op, (1u << op.t) | (1u << op.t2));
+      break;
+  }
+  plan->ops.push_back(op);
+}
+
+}  // namespace
+
+int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
+               const PlanOptions& opt, FusedPlan* plan) {
+  if (n < 1 || n > 36) return set_error(QOA_ERR_ARG, "n_qubits %u out of range", n);
+  uint32_t k = opt.tile_qubits ? opt.tile_qubits : 12;
+  if (k > 13) return set_error(QOA_ERR_ARG, "tile_qubits %u exceeds 13 (128 KiB of LDS)", k);
+  if (k < 4) k = 4;
+  if (k > n) k = n;
+  uint32_t m = opt.low_qubits ? opt.low_qubits : 3;
+  if (m + 2 > k) m = k >= 2 ? k - 2 : 0;
+
+  std::vector<FOp> ops;
+  ops.reserve(ng);
+  plan->n = n;
+  plan->k = k;
+  plan->passes.clear();
+  plan->ops.clear();
+  plan->pool.clear();
+  plan->order.clear();
+  for (uint32_t i = 0; i < ng; ++i) {
+    size_t before = ops.size();
+    int rc = canonical(n, g[i], mats, opt.flags, &ops, i);
+    if (rc) return rc;
+    if (ops.size() == before) plan->order.push_back(i);  // a no-op: position is irrelevant
+  }
+  const uint64_t allq = n >= 64 ? ~0ULL : ((1ULL << n) - 1);
+  const uint64_t low = m ? ((1ULL << m) - 1) : 0;
+  std::vector<uint32_t> rem(ops.size());
+  for (uint32_t i = 0; i < ops.size(); ++i) rem[i] = i;
+
+  while (!rem.empty()) {
+    uint64_t S;
+    if (k >= n) {
+      S = allq;
+    } else {
+      S = low | ops[rem[0]].active;
+      int cur = simulate(ops, rem, S, allq);
+      while ((uint32_t)__builtin_popcountll(S) < k) {
+        int best = -1, bests = -1;
+        for (uint32_t q = 0; q < n; ++q) {
+          if (S & bit(q)) continue;
+          const int sc = simulate(ops, rem, S | bit(q), allq);
+          if (sc > bests) {
+            bests = sc;
+            best = (int)q;
+          }
+        }
+        if (bests <= cur && (uint32_t)__builtin_popcountll(S) + 2 <= k) {
+          // no single qubit helps: try the best pair (two-qubit gates need both)
+          int b1 = -1, b2 = -1, bp = cur;
+          for (uint32_t q = 0; q < n; ++q) {
+            if (S & bit(q)) continue;
+            for (uint32_t r = q + 1; r < n; ++r) {
+              if (S & bit(r)) continue;
+              const int sc = simulate(ops, rem, S | bit(q) | bit(r), allq);
+              if (sc > bp) {
+                bp = sc;
+                b1 = (int)q;
+                b2 = (int)r;
+              }
+            }
+          }
+          if (b1 >= 0) {
+            S |= bit(b1) | bit(b2);
+            cur = bp;
+            continue;
+          }
+        }
+        S |= bit(best);
+        cur = bests;
+      }
+    }
+    PassPlan pp{};
+    pp.k = k;
+    int j = 0;
+    for (uint32_t q = 0; q < n; ++q)
+      if (S & bit(q)) pp.spos[j++] = (uint8_t)q;
+    pp.op_begin = (uint32_t)plan->ops.size();
+    uint64_t blocked = 0;
+    std::vector<uint32_t> next;
+    next.reserve(rem.size());
+    for (uint32_t i : rem) {
+      const FOp& f = ops[i];
+      if ((f.all & blocked) == 0 && (f.active & ~S) == 0) {
+        emit(f, S, pp.spos, (int)k, plan);
+        plan->order.push_back(f.gi);
+      } else {
+        blocked |= f.all;
+        next.push_back(i);
+      }
+    }
+    pp.op_end = (uint32_t)plan->ops.size();
+    plan->passes.push_back(pp);
+    rem.swap(next);
+  }
+  return QOA_OK;
+}
+
+int plan_order(uint32_t n, const qoa_gate* g, uint32_t ng, const PlanOptions& opt,
+               std::vector<uint32_t>* order, uint32_t* n_passes) {
+  FusedPlan plan;
+  // matrices are only needed for coefficients, not for the schedule
+  std::vector<double> zeros;
+  uint64_t maxaux = 0;
+  for (uint32_t i = 0; i < ng; ++i)
+    if (g[i].op == QOA_OP_U2 || g[i].op == QOA_OP_U4) maxaux = std::max<uint64_t>(maxaux, g[i].aux + 1);
+  zeros.assign(32 * maxaux + 32, 0.0);
+  int rc = build_plan(n, g, ng, zeros.data(), opt, &plan);
+  if (rc) return rc;
+  if (order) *order = plan.order;
+  if (n_passes) *n_passes = (uint32_t)plan.passes.size();
+  return QOA_OK;
+}
+
+}  // namespace qoa

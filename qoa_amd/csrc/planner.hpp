@@ -1,0 +1,54 @@
+// planner.hpp — gate list -> LDS-tile passes (host side of the fused path).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "../../include/qoa_hip.h"
+
+namespace qoa {
+
+// Tile-local op kinds (device op program).
+enum TileKind : uint8_t {
+  TK_H = 0,       // a' = nf*(a+b), b' = nf*(a-b)                      c[0..1] = nf
+  TK_M = 1,       // a' = m00 a + m01 b, b' = m10 a + m11 b            c[0..7]
+  TK_X = 2,       // swap
+  TK_Y = 3,       // a' = b*(-i), b' = a*(+i)
+  TK_DNEG = 4,    // v = v.scale(-1) where (l & lmask) == lmask
+  TK_DPHASE = 5,  // v *= c[0..1]   where (l & lmask) == lmask
+  TK_RZ = 6,      // v *= bit t ? c[2..3] : c[0..1]
+  TK_SWAP = 7,    // exchange (l|m2, l|m1) for l with both bits clear (and lmask set)
+  TK_MAT4 = 8     // 4x4 on local (t, t2) = (q1, q2); coefficients at pool[aux]
+};
+
+// One op of a pass program. `ins` holds the sorted local bit positions into
+// which a zero is deposited to enumerate the visited indices (nins == 0xFF:
+// too many bits, scan every index with the predicate instead).
+struct alignas(16) TileOp {
+  uint8_t kind, t, t2, nins;
+  uint8_t ins[4];
+  uint32_t lmask;
+  uint32_t aux;
+  uint64_t gmask, gval;  // op applies to a tile iff (tile_base & gmask) == gval
+  double c[8];
+};
+static_assert(sizeof(TileOp) == 96, "TileOp layout");
+
+struct PassPlan {
+  uint32_t k;                 // tile qubits
+  uint8_t spos[16];           // physical positions of the tile qubits, ascending
+  uint32_t op_begin, op_end;  // range in the flat op array
+};
+
+struct FusedPlan {
+  uint32_t n = 0, k = 0;
+  std::vector<PassPlan> passes;
+  std::vector<TileOp> ops;
+  std::vector<double> pool;     // MAT4 coefficients
+  std::vector<uint32_t> order;  // executed order of gate indices
+};
+
+struct PlanOptions;
+int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
+               const PlanOptions& opt, FusedPlan* plan);
+
+}  // namespace qoa

@@ -63,6 +63,7 @@ struct qoa_state {
   void* amps;          // device buffer (double2 or float2)
   hipStream_t stream;
   double* red;         // reduction scratch: partials + result slots
+  void* fused_cache;   // last fused plan and its device copy (fused.hip)
   // QuantumState-level fields (src/runtime/quantum_state.rs:37-53)
   int noise_kind;
   double noise_p;
@@ -86,6 +87,10 @@ constexpr int kSlotBase = 2 * kRedBlocks;
 constexpr int kRedDoubles = kSlotBase + 16;
 
 bool valid(const qoa_state* s);
+
+// 32-bit index arithmetic unless the state needs more (or it is forced off).
+extern bool g_force_index64;
+inline bool use32(const qoa_state* s) { return s->n <= 31 && !g_force_index64; }
 
 // launch bracketing for timing
 void timing_begin(qoa_state* s, int kind, double bytes);
@@ -131,6 +136,7 @@ int plan_order(uint32_t n, const qoa_gate* g, uint32_t ng, const PlanOptions& op
                std::vector<uint32_t>* order, uint32_t* n_passes);
 int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
               const PlanOptions& opt);
+void fused_cache_free(qoa_state* s);
 
 // one gate with layer-1 semantics (api.cpp)
 int apply_gate(qoa_state* s, const qoa_gate* g, const double* mats, uint32_t flags);

@@ -1,0 +1,107 @@
+"""Fused LDS passes: bit-identical to the per-gate run of the planned order,
+within 1e-12 of the original order, at every tile size."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from kat_util import random_state
+from qoa_amd import StateVector, plan_fused
+from qoa_amd import _native
+from qoa_amd import workloads as W
+from qoa_amd._native import QOA_CZ_REF_MASKS
+from test_planner import mixed_circuit
+
+pytestmark = pytest.mark.gpu
+
+
+def fidelity(a, b):
+    return abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+@pytest.mark.parametrize("tile,low", [(4, 2), (6, 3), (8, 3), (12, 3), (12, 4), (13, 3)])
+def test_c1_fused_matches_planned_order_bit_exact(tile, low):
+    n = 16
+    g = W.layered_random(n, 20, seed=1)
+    order, passes = plan_fused(n, g, tile_qubits=tile, low_qubits=low)
+    ref = po.state(n)
+    po.run_gates(ref, g, order=order)
+    orig = po.state(n)
+    po.run_gates(orig, g)
+    with StateVector(n) as sv:
+        sv.run_fused(g, tile_qubits=tile, low_qubits=low)
+        out = sv.download()
+    assert np.array_equal(out, ref), f"tile={tile} passes={passes}"
+    assert np.max(np.abs(out - orig)) <= 1e-12
+    assert fidelity(out, orig) >= 1 - 1e-12
+
+
+@pytest.mark.parametrize("n,tile", [(10, 4), (10, 6), (12, 8), (14, 12)])
+@pytest.mark.parametrize("flags", [0, QOA_CZ_REF_MASKS])
+def test_mixed_gate_surface_fused(n, tile, flags):
+    g, mats = mixed_circuit(n, 400, seed=n * 7 + tile)
+    order, _ = plan_fused(n, g, tile_qubits=tile, flags=flags)
+    a0 = random_state(n, seed=n)
+    ref = a0.copy()
+    po.run_gates(ref, g, flags=flags, matrices=mats, order=order)
+    with StateVector(n) as sv:
+        sv.upload(a0)
+        sv.run_fused(g, tile_qubits=tile, flags=flags, matrices=mats)
+        out = sv.download()
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 9, 12])
+def test_whole_state_single_tile(n):
+    g = W.layered_random(n, 12, seed=n, ladders=True)
+    ref = po.state(n)
+    po.run_gates(ref, g)
+    with StateVector(n) as sv:
+        sv.run_fused(g)
+        assert np.array_equal(sv.download(), ref)
+
+
+def test_index64_paths_match():
+    """The 64-bit index kernels (taken by states of 32+ qubits) give the same
+    results as the 32-bit ones."""
+    lib = _native.load()
+    n = 14
+    g = W.layered_random(n, 8, seed=3, ladders=True)
+    gm, mats = mixed_circuit(n, 200, seed=5)
+    outs = []
+    try:
+        for force in (0, 1):
+            _native.check(lib.qoa_set_option(b"index64", force))
+            with StateVector(n) as sv:
+                sv.run_gates(g)
+                sv.run_fused(g, tile_qubits=9)
+                sv.run_gates(gm, matrices=mats)
+                sv.run_fused(gm, tile_qubits=7, matrices=mats)
+                sv.apply_reset(3)
+                s, _ = sv.normalize()
+                o, _, _ = sv.measure_with(5, 0.3)
+                outs.append((sv.download(), sv.probabilities(), s, o))
+    finally:
+        lib.qoa_set_option(b"index64", 0)
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2] and outs[0][3] == outs[1][3]
+
+
+def test_c2_28q_fused_vs_per_gate_full_size():
+    """The full 28-qubit, 3164-gate circuit: fused passes against per-gate
+    sweeps on the device (size-independent checks: unit norm, agreement
+    <= 1e-12 in every amplitude, fidelity >= 1-1e-12)."""
+    n = 28
+    g = W.layered_random(n, 40, ladders=True)
+    with StateVector(n) as a, StateVector(n) as b:
+        a.run_gates(g)
+        b.run_fused(g)
+        na, nb = a.norm_sqr(), b.norm_sqr()
+        assert abs(na - 1) < 1e-10 and abs(nb - 1) < 1e-10
+        # compare in chunks on the host to bound memory
+        da, db = a.download(), b.download()
+    d = np.max(np.abs(da - db))
+    f = fidelity(da, db)
+    del da, db
+    assert d <= 1e-12, d
+    assert f >= 1 - 1e-12, f

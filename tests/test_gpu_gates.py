@@ -71,7 +71,13 @@ def test_one_qubit_all_targets(n, name, mk):
             sv.upload(a0)
             run_l1_hip(sv, name, mk(q))
             out = sv.download()
-            assert np.array_equal(out, ref), f"{name} n={n} q={q}"
+            if name == "reset" and n > 2:
+                # the bit-0 norm is a reduction: its summation order differs
+                # from the CPU loop at the ulp level, the collapse itself is exact
+                assert np.array_equal(out == 0, ref == 0), f"{name} n={n} q={q}"
+                assert np.max(np.abs(out - ref)) <= 4e-16, f"{name} n={n} q={q}"
+            else:
+                assert np.array_equal(out, ref), f"{name} n={n} q={q}"
 
 
 @pytest.mark.parametrize("n", [2, 4, 8, 13])

@@ -1,0 +1,88 @@
+"""The fused-pass planner (host code, no GPU): its executed order is a valid
+reordering of the circuit, checked by running the oracle in both orders."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from kat_util import random_state
+from qoa_amd import plan_fused
+from qoa_amd import workloads as W
+from qoa_amd._native import GATE_DTYPE, OP, QOA_CZ_REF_MASKS
+
+
+def mixed_circuit(n: int, count: int, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """Every gate kind of the surface, random operands (some out of range)."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    rows = []
+    kinds = ["H", "X", "Y", "Z", "S", "T", "PHASE", "RX", "RY", "RZ", "CNOT", "CZ", "CPHASE",
+             "SWAP", "CSWAP", "TOFFOLI", "CY", "U2", "U4"]
+    for _ in range(count):
+        k = kinds[rng.integers(len(kinds))]
+        qs = rng.choice(n, size=3, replace=False)
+        q0, q1, q2 = (int(x) for x in qs)
+        th = float(rng.uniform(0, 2 * np.pi))
+        aux = 0
+        if k in ("H", "RZ", "X") and rng.random() < 0.05:
+            q0 = n + 1  # out of range: no-op (RZ: global phase)
+        if k == "U2":
+            u = np.linalg.qr(rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2)))[0]
+            aux = len(mats) // 1
+            m = np.zeros(16, dtype=np.complex128)
+            m[:4] = u.reshape(4)
+            mats.append(m)
+        if k == "U4":
+            u = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]
+            aux = len(mats)
+            mats.append(u.reshape(16))
+        rows.append((OP[k], q0, q1, q2, th, aux))
+    g = np.array(rows, dtype=GATE_DTYPE)
+    # U2 reads 8 doubles at 8*aux and U4 32 doubles at 32*aux: lay matrices out
+    # so both indexings hit the right entries (one 16-complex slot per matrix).
+    table = np.zeros(16 * max(1, len(mats)), dtype=np.complex128)
+    for i, m in enumerate(mats):
+        table[16 * i:16 * i + 16] = m
+    for i in range(len(g)):
+        if g[i]["op"] == OP["U2"]:
+            g[i]["aux"] = 4 * int(g[i]["aux"])
+    return g, table
+
+
+@pytest.mark.parametrize("tile", [4, 6, 12])
+def test_plan_is_permutation_and_valid_reordering(tile):
+    n = 10
+    g, mats = mixed_circuit(n, 300, seed=tile)
+    order, passes = plan_fused(n, g, tile_qubits=tile)
+    assert sorted(order.tolist()) == list(range(len(g)))
+    assert passes >= 1
+    a0 = random_state(n, 3)
+    ref = a0.copy()
+    po.run_gates(ref, g, matrices=mats)
+    alt = a0.copy()
+    po.run_gates(alt, g, matrices=mats, order=order)
+    assert np.max(np.abs(alt - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("flags", [0, QOA_CZ_REF_MASKS])
+def test_plan_c2_shape(flags):
+    n = 14
+    g = W.layered_random(n, 15, seed=5, ladders=True)
+    order, passes = plan_fused(n, g, tile_qubits=8, flags=flags)
+    ref = po.state(n)
+    po.run_gates(ref, g, flags=flags)
+    alt = po.state(n)
+    po.run_gates(alt, g, flags=flags, order=order)
+    assert np.max(np.abs(alt - ref)) < 1e-12
+
+
+def test_plan_whole_state_in_one_tile():
+    g = W.layered_random(9, 20, seed=1, ladders=True)
+    order, passes = plan_fused(9, g, tile_qubits=12)
+    assert passes == 1 and order.tolist() == list(range(len(g)))
+
+
+def test_plan_pass_counts_28q():
+    """The 3164-gate 28-qubit circuit needs far fewer passes than gates."""
+    g = W.layered_random(28, 40, ladders=True)
+    _, passes = plan_fused(28, g, tile_qubits=12, low_qubits=3)
+    assert passes < len(g) / 8, passes
