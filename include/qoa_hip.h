@@ -85,6 +85,9 @@ typedef struct qoa_gate {
 } qoa_gate;
 
 /* Flags for circuit execution. */
+#define QOA_FUSE_LDS_ONLY 2u /* fused passes keep the whole tile in LDS and
+                                apply every gate there (no register-resident
+                                segments); for comparison and testing.      */
 #define QOA_CZ_REF_MASKS 1u  /* CZ/CPHASE use the integer c|t as the bit mask,
                                 as QuantumState::apply_cz does
                                 (src/runtime/quantum_state.rs:338, 356-361).

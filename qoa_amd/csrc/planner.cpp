@@ -197,7 +197,8 @@ void set_ins(TileOp* op, uint32_t bits) {
     if (bits & (1u << j)) op->ins[op->nins++] = (uint8_t)j;
 }
 
-void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, FusedPlan* plan) {
+void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<TileOp>* out,
+          std::vector<double>* pool) {
   TileOp op{};
   const uint64_t pin = f.pmask & S, pout = f.pmask & ~S;
   op.lmask = local_bits(pin, spos, k);
@@ -229,8 +230,8 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, FusedPlan* plan)
         o1.gval = bit(f.rzq);
         o1.c[0] = f.c[2];
         o1.c[1] = f.c[3];
-        plan->ops.push_back(o0);
-        plan->ops.push_back(o1);
+        out->push_back(o0);
+        out->push_back(o1);
         return;
       }
       break;
@@ -244,12 +245,141 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, FusedPlan* plan)
       op.kind = TK_MAT4;
       op.t = (uint8_t)loc_of(spos, k, f.t);
       op.t2 = (uint8_t)loc_of(spos, k, f.t2);
-      op.aux = (uint32_t)plan->pool.size();
-      plan->pool.insert(plan->pool.end(), f.m4, f.m4 + 32);
+      op.aux = (uint32_t)pool->size();
+      pool->insert(pool->end(), f.m4, f.m4 + 32);
       set_ins(&op, (1u << op.t) | (1u << op.t2));
       break;
   }
-  plan->ops.push_back(op);
+  out->push_back(op);
+}
+
+// ---- register segments inside a pass ------------------------------------------
+// Each thread of the register-resident kernel holds the kRegAmps amplitudes
+// whose local indices differ in the register bits R. Ops that mix only bits
+// of R (and every diagonal op) run in registers; moving to a new R costs one
+// LDS round trip (a TK_LAYOUT op). The same greedy list scheduling as the
+// pass planner picks R: ops on disjoint bits commute, so the emitted order
+// stays a valid reordering. The pass starts and ends in the layout that
+// holds the top kRegBits local bits in registers (coalesced HBM access).
+struct SegOp {
+  uint32_t act, all;
+};
+
+inline SegOp seg_masks(const TileOp& op) {
+  switch (op.kind) {
+    case TK_H: case TK_M: case TK_X: case TK_Y:
+      return {1u << op.t, (1u << op.t) | op.lmask};
+    case TK_SWAP: case TK_MAT4: {
+      const uint32_t a = (1u << op.t) | (1u << op.t2);
+      return {a, a | op.lmask};
+    }
+    case TK_RZ: return {0u, 1u << op.t};
+    default: return {0u, op.lmask};
+  }
+}
+
+int seg_sim(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint32_t R,
+            uint32_t allb) {
+  uint32_t blocked = 0;
+  int cnt = 0;
+  for (uint32_t i : rem) {
+    if ((m[i].all & blocked) == 0 && (m[i].act & ~R) == 0) {
+      ++cnt;
+    } else {
+      blocked |= m[i].all;
+      if ((blocked & allb) == allb) break;
+    }
+  }
+  return cnt;
+}
+
+uint32_t seg_choose(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint32_t seed,
+                    uint32_t k, uint32_t allb) {
+  uint32_t R = seed;
+  int cur = seg_sim(m, rem, R, allb);
+  while (__builtin_popcount(R) < kRegBits) {
+    int best = -1, bests = -1;
+    for (uint32_t b = 0; b < k; ++b) {
+      if (R & (1u << b)) continue;
+      const int sc = seg_sim(m, rem, R | (1u << b), allb);
+      if (sc > bests) { bests = sc; best = (int)b; }
+    }
+    if (bests <= cur && __builtin_popcount(R) + 2 <= kRegBits) {
+      int b1 = -1, b2 = -1, bp = cur;
+      for (uint32_t b = 0; b < k; ++b) {
+        if (R & (1u << b)) continue;
+        for (uint32_t c = b + 1; c < k; ++c) {
+          if (R & (1u << c)) continue;
+          const int sc = seg_sim(m, rem, R | (1u << b) | (1u << c), allb);
+          if (sc > bp) { bp = sc; b1 = (int)b; b2 = (int)c; }
+        }
+      }
+      if (b1 >= 0) {
+        R |= (1u << b1) | (1u << b2);
+        cur = bp;
+        continue;
+      }
+    }
+    R |= 1u << best;
+    cur = bests;
+  }
+  return R;
+}
+
+TileOp layout_op(uint32_t R) {
+  TileOp op{};
+  op.kind = TK_LAYOUT;
+  int j = 0;
+  for (int b = 0; b < 32 && j < kRegBits; ++b)
+    if (R & (1u << b)) op.ins[j++] = (uint8_t)b;
+  return op;
+}
+
+inline uint8_t slot_of(uint32_t R, uint32_t bitpos) {
+  return (uint8_t)__builtin_popcount(R & ((1u << bitpos) - 1u));
+}
+
+void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& gis, uint32_t k,
+                  std::vector<TileOp>* out, std::vector<uint32_t>* order) {
+  const uint32_t L0 = ((1u << kRegBits) - 1u) << (k - kRegBits);
+  const uint32_t allb = (1u << k) - 1u;
+  std::vector<SegOp> m(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i) m[i] = seg_masks(ops[i]);
+  std::vector<uint32_t> rem(ops.size());
+  for (uint32_t i = 0; i < rem.size(); ++i) rem[i] = i;
+  uint32_t R = L0;
+  uint32_t last_gi = 0xFFFFFFFFu;
+  while (!rem.empty()) {
+    const int cur = seg_sim(m, rem, R, allb);
+    const uint32_t seed = cur == 0 ? m[rem[0]].act : 0u;
+    const uint32_t nr = seg_choose(m, rem, seed, k, allb);
+    if (nr != R && (cur == 0 || seg_sim(m, rem, nr, allb) > cur)) {
+      R = nr;
+      out->push_back(layout_op(R));
+    }
+    uint32_t blocked = 0;
+    std::vector<uint32_t> next;
+    next.reserve(rem.size());
+    for (uint32_t i : rem) {
+      if ((m[i].all & blocked) == 0 && (m[i].act & ~R) == 0) {
+        TileOp op = ops[i];
+        if (m[i].act) {
+          op.ins[0] = slot_of(R, op.t);
+          if (op.kind == TK_SWAP || op.kind == TK_MAT4) op.ins[1] = slot_of(R, op.t2);
+        }
+        out->push_back(op);
+        if (gis[i] != last_gi) {
+          order->push_back(gis[i]);
+          last_gi = gis[i];
+        }
+      } else {
+        blocked |= m[i].all;
+        next.push_back(i);
+      }
+    }
+    rem.swap(next);
+  }
+  if (R != L0) out->push_back(layout_op(L0));
 }
 
 }  // namespace
@@ -330,19 +460,30 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
     int j = 0;
     for (uint32_t q = 0; q < n; ++q)
       if (S & bit(q)) pp.spos[j++] = (uint8_t)q;
+    pp.reg_mode = (k >= 8 && !(opt.flags & QOA_FUSE_LDS_ONLY)) ? 1u : 0u;
     pp.op_begin = (uint32_t)plan->ops.size();
     uint64_t blocked = 0;
     std::vector<uint32_t> next;
     next.reserve(rem.size());
+    std::vector<TileOp> pass_ops;
+    std::vector<uint32_t> pass_gi;
     for (uint32_t i : rem) {
       const FOp& f = ops[i];
       if ((f.all & blocked) == 0 && (f.active & ~S) == 0) {
-        emit(f, S, pp.spos, (int)k, plan);
-        plan->order.push_back(f.gi);
+        const size_t before = pass_ops.size();
+        emit(f, S, pp.spos, (int)k, &pass_ops, &plan->pool);
+        for (size_t e = before; e < pass_ops.size(); ++e) pass_gi.push_back(f.gi);
       } else {
         blocked |= f.all;
         next.push_back(i);
       }
+    }
+    if (pp.reg_mode) {
+      segment_pass(pass_ops, pass_gi, k, &plan->ops, &plan->order);
+    } else {
+      plan->ops.insert(plan->ops.end(), pass_ops.begin(), pass_ops.end());
+      for (size_t e = 0; e < pass_gi.size(); ++e)
+        if (e == 0 || pass_gi[e] != pass_gi[e - 1]) plan->order.push_back(pass_gi[e]);
     }
     pp.op_end = (uint32_t)plan->ops.size();
     plan->passes.push_back(pp);

@@ -17,8 +17,16 @@ enum TileKind : uint8_t {
   TK_DPHASE = 5,  // v *= c[0..1]   where (l & lmask) == lmask
   TK_RZ = 6,      // v *= bit t ? c[2..3] : c[0..1]
   TK_SWAP = 7,    // exchange (l|m2, l|m1) for l with both bits clear (and lmask set)
-  TK_MAT4 = 8     // 4x4 on local (t, t2) = (q1, q2); coefficients at pool[aux]
+  TK_MAT4 = 8,    // 4x4 on local (t, t2) = (q1, q2); coefficients at pool[aux]
+  TK_LAYOUT = 9   // register passes: ins[0..3] = local bits held in registers
 };
+
+// Register-resident passes: each thread holds kRegAmps amplitudes of the tile
+// whose local indices differ in kRegBits "register bits"; the other bits
+// come from the thread index. In register mode ins[0] (and ins[1]) of a 2x2,
+// SWAP or MAT4 op hold the register slot of its target bit(s).
+constexpr int kRegBits = 4;
+constexpr int kRegAmps = 1 << kRegBits;
 
 // One op of a pass program. `ins` holds the sorted local bit positions into
 // which a zero is deposited to enumerate the visited indices (nins == 0xFF:
@@ -35,6 +43,7 @@ static_assert(sizeof(TileOp) == 96, "TileOp layout");
 
 struct PassPlan {
   uint32_t k;                 // tile qubits
+  uint32_t reg_mode;          // 1: register-resident kernel, 0: LDS-resident kernel
   uint8_t spos[16];           // physical positions of the tile qubits, ascending
   uint32_t op_begin, op_end;  // range in the flat op array
 };

@@ -9,6 +9,9 @@ from qoa_amd import StateVector, plan_fused
 from qoa_amd import _native
 from qoa_amd import workloads as W
 from qoa_amd._native import QOA_CZ_REF_MASKS
+
+QOA_FUSE_LDS_ONLY = 2
+MODES = [0, QOA_FUSE_LDS_ONLY]  # register-resident segments (default) / LDS-resident
 from test_planner import mixed_circuit
 
 pytestmark = pytest.mark.gpu
@@ -18,25 +21,27 @@ def fidelity(a, b):
     return abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
 
 
-@pytest.mark.parametrize("tile,low", [(4, 2), (6, 3), (8, 3), (12, 3), (12, 4), (13, 3)])
-def test_c1_fused_matches_planned_order_bit_exact(tile, low):
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("tile,low", [(4, 2), (6, 3), (8, 3), (10, 3), (12, 3), (12, 4), (13, 3)])
+def test_c1_fused_matches_planned_order_bit_exact(tile, low, mode):
     n = 16
     g = W.layered_random(n, 20, seed=1)
-    order, passes = plan_fused(n, g, tile_qubits=tile, low_qubits=low)
+    order, passes = plan_fused(n, g, tile_qubits=tile, low_qubits=low, flags=mode)
     ref = po.state(n)
     po.run_gates(ref, g, order=order)
     orig = po.state(n)
     po.run_gates(orig, g)
     with StateVector(n) as sv:
-        sv.run_fused(g, tile_qubits=tile, low_qubits=low)
+        sv.run_fused(g, tile_qubits=tile, low_qubits=low, flags=mode)
         out = sv.download()
     assert np.array_equal(out, ref), f"tile={tile} passes={passes}"
     assert np.max(np.abs(out - orig)) <= 1e-12
     assert fidelity(out, orig) >= 1 - 1e-12
 
 
-@pytest.mark.parametrize("n,tile", [(10, 4), (10, 6), (12, 8), (14, 12)])
-@pytest.mark.parametrize("flags", [0, QOA_CZ_REF_MASKS])
+@pytest.mark.parametrize("n,tile", [(10, 4), (10, 6), (12, 8), (14, 12), (15, 10)])
+@pytest.mark.parametrize("flags", [0, QOA_CZ_REF_MASKS, QOA_FUSE_LDS_ONLY,
+                                   QOA_FUSE_LDS_ONLY | QOA_CZ_REF_MASKS])
 def test_mixed_gate_surface_fused(n, tile, flags):
     g, mats = mixed_circuit(n, 400, seed=n * 7 + tile)
     order, _ = plan_fused(n, g, tile_qubits=tile, flags=flags)
@@ -50,7 +55,7 @@ def test_mixed_gate_surface_fused(n, tile, flags):
     assert np.array_equal(out, ref)
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 9, 12])
+@pytest.mark.parametrize("n", [1, 2, 5, 8, 9, 12, 13])
 def test_whole_state_single_tile(n):
     g = W.layered_random(n, 12, seed=n, ladders=True)
     ref = po.state(n)
@@ -87,7 +92,8 @@ def test_index64_paths_match():
     assert outs[0][2] == outs[1][2] and outs[0][3] == outs[1][3]
 
 
-def test_c2_28q_fused_vs_per_gate_full_size():
+@pytest.mark.parametrize("mode", MODES)
+def test_c2_28q_fused_vs_per_gate_full_size(mode):
     """The full 28-qubit, 3164-gate circuit: fused passes against per-gate
     sweeps on the device (size-independent checks: unit norm, agreement
     <= 1e-12 in every amplitude, fidelity >= 1-1e-12)."""
@@ -95,7 +101,7 @@ def test_c2_28q_fused_vs_per_gate_full_size():
     g = W.layered_random(n, 40, ladders=True)
     with StateVector(n) as a, StateVector(n) as b:
         a.run_gates(g)
-        b.run_fused(g)
+        b.run_fused(g, flags=mode)
         na, nb = a.norm_sqr(), b.norm_sqr()
         assert abs(na - 1) < 1e-10 and abs(nb - 1) < 1e-10
         # compare in chunks on the host to bound memory

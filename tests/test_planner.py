@@ -48,11 +48,12 @@ def mixed_circuit(n: int, count: int, seed: int) -> tuple[np.ndarray, np.ndarray
     return g, table
 
 
-@pytest.mark.parametrize("tile", [4, 6, 12])
-def test_plan_is_permutation_and_valid_reordering(tile):
+@pytest.mark.parametrize("flags", [0, 2])
+@pytest.mark.parametrize("tile", [4, 6, 8, 12])
+def test_plan_is_permutation_and_valid_reordering(tile, flags):
     n = 10
     g, mats = mixed_circuit(n, 300, seed=tile)
-    order, passes = plan_fused(n, g, tile_qubits=tile)
+    order, passes = plan_fused(n, g, tile_qubits=tile, flags=flags)
     assert sorted(order.tolist()) == list(range(len(g)))
     assert passes >= 1
     a0 = random_state(n, 3)
@@ -77,8 +78,15 @@ def test_plan_c2_shape(flags):
 
 def test_plan_whole_state_in_one_tile():
     g = W.layered_random(9, 20, seed=1, ladders=True)
-    order, passes = plan_fused(9, g, tile_qubits=12)
+    order, passes = plan_fused(9, g, tile_qubits=12, flags=2)  # LDS-resident: no reordering
     assert passes == 1 and order.tolist() == list(range(len(g)))
+    order, passes = plan_fused(9, g, tile_qubits=12)  # register segments reorder in the tile
+    assert passes == 1 and sorted(order.tolist()) == list(range(len(g)))
+    ref = po.state(9)
+    po.run_gates(ref, g)
+    alt = po.state(9)
+    po.run_gates(alt, g, order=order)
+    assert np.max(np.abs(alt - ref)) < 1e-12
 
 
 def test_plan_pass_counts_28q():
