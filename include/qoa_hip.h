@@ -239,17 +239,35 @@ int qoa_qs_measure_all(qoa_state* s, uint8_t* out, uint32_t n_out);
 /* apply_noise + normalize as run after every gate (quantum_state.rs:839-907). */
 int qoa_qs_apply_noise(qoa_state* s);
 
-/* ---- trajectories and batches (many states, one launch per layer) ------- */
+/* ---- batches: many independent circuits, one launch per pass round ------- */
 typedef struct qoa_batch qoa_batch;
 typedef struct qoa_job_desc {
-  uint32_t n_qubits;     /* 1..30                                         */
+  uint32_t n_qubits;     /* 4..30                                         */
   uint32_t n_gates;      /* gates of this job in the shared gate array     */
   uint64_t gate_offset;  /* first gate index in the shared gate array      */
   uint64_t noise_seed;   /* Philox key for the job's noise trajectories    */
   uint32_t trajectories; /* >= 1; each is an independent state             */
-  uint32_t precision;    /* QOA_C128 or QOA_C64                            */
+  uint32_t precision;    /* QOA_C128                                       */
   double p1, p2;         /* Pauli error prob after 1q / 2q gates (0 = none)*/
 } qoa_job_desc;
+/* Plans every job (fused register passes, tile = min(opts->tile_qubits or
+   12, n)) and allocates every instance's state in one device buffer. Fails
+   with QOA_ERR_OOM when the states do not fit. Gates use layer-1 semantics
+   (opts->flags may hold QOA_CZ_REF_MASKS). */
+int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* gates,
+                     uint64_t n_gates_total, const double* matrices,
+                     const qoa_fuse_opts* opts, qoa_batch** out);
+/* Runs every instance from |0...0>: one launch per round (round r = pass r
+   of every job that has it). Asynchronous; see qoa_batch_synchronize. */
+int qoa_batch_run(qoa_batch* b);
+/* Waits for the last run; *run_ms = its device time (HIP events). */
+int qoa_batch_synchronize(qoa_batch* b, double* run_ms);
+int qoa_batch_info(qoa_batch* b, uint32_t* n_rounds, uint64_t* total_amps, double* alg_bytes);
+int qoa_batch_download(qoa_batch* b, uint32_t job, uint32_t traj, double* amps,
+                       uint64_t n_amps);
+/* Executed gate order of a job (for bit-exact checks against the oracle). */
+int qoa_batch_order(qoa_batch* b, uint32_t job, uint32_t* order, uint32_t n_order);
+int qoa_batch_destroy(qoa_batch* b);
 
 #ifdef __cplusplus
 }

@@ -160,3 +160,14 @@ def check(rc: int) -> int:
 def ptr(a: np.ndarray) -> C.c_void_p:
     assert a.flags["C_CONTIGUOUS"]
     return C.c_void_p(a.ctypes.data)
+
+
+_SIGS.update({
+    "qoa_batch_create": ([C.POINTER(QoaJobDesc), U32, P, U64, P, P, C.POINTER(P)], I32),
+    "qoa_batch_run": ([P], I32),
+    "qoa_batch_synchronize": ([P, DP], I32),
+    "qoa_batch_info": ([P, C.POINTER(U32), C.POINTER(U64), DP], I32),
+    "qoa_batch_download": ([P, U32, U32, P, U64], I32),
+    "qoa_batch_order": ([P, U32, P, U32], I32),
+    "qoa_batch_destroy": ([P], I32),
+})

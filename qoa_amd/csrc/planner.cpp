@@ -460,7 +460,9 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
     int j = 0;
     for (uint32_t q = 0; q < n; ++q)
       if (S & bit(q)) pp.spos[j++] = (uint8_t)q;
-    pp.reg_mode = (k >= 8 && !(opt.flags & QOA_FUSE_LDS_ONLY)) ? 1u : 0u;
+    pp.reg_mode =
+        ((k >= 8 || (opt.flags & kPlanForceReg)) && k >= (uint32_t)kRegBits &&
+         !(opt.flags & QOA_FUSE_LDS_ONLY)) ? 1u : 0u;
     pp.op_begin = (uint32_t)plan->ops.size();
     uint64_t blocked = 0;
     std::vector<uint32_t> next;

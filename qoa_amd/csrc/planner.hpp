@@ -27,6 +27,9 @@ enum TileKind : uint8_t {
 // SWAP or MAT4 op hold the register slot of its target bit(s).
 constexpr int kRegBits = 4;
 constexpr int kRegAmps = 1 << kRegBits;
+// PlanOptions.flags bit (internal): register mode for every tile of at least
+// kRegBits qubits (the batched kernel runs small whole-state tiles this way).
+constexpr uint32_t kPlanForceReg = 1u << 31;
 
 // One op of a pass program. `ins` holds the sorted local bit positions into
 // which a zero is deposited to enumerate the visited indices (nins == 0xFF:

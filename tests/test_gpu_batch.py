@@ -1,0 +1,56 @@
+"""Batched circuits (one launch per pass round): every job bit-identical to
+the oracle in its planned order, within 1e-12 of its original order."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from qoa_amd import workloads as W
+from qoa_amd.batch import Batch
+from test_planner import mixed_circuit
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("tile", [8, 10, 12])
+def test_ansatz_batch_matches_oracle(tile):
+    jobs = []
+    for j in range(24):
+        n, depth, g = W.ansatz_job(900 + j, n_min=4, n_max=15, d_min=2, d_max=12)
+        jobs.append((n, g))
+    with Batch(jobs, tile_qubits=tile) as b:
+        b.run()
+        ms = b.synchronize()
+        assert ms > 0
+        info = b.info()
+        assert info["rounds"] >= 1
+        first = [b.download(j) for j in range(len(jobs))]
+        for j, (n, g) in enumerate(jobs):
+            out = first[j]
+            ref = po.state(n)
+            po.run_gates(ref, g, order=b.order(j))
+            assert np.array_equal(out, ref), f"job {j} n={n}"
+            orig = po.state(n)
+            po.run_gates(orig, g)
+            assert np.max(np.abs(out - orig)) <= 1e-12
+        # a second run starts again from |0...0>
+        b.run()
+        b.synchronize()
+        for j in range(len(jobs)):
+            assert np.array_equal(b.download(j), first[j])
+
+
+def test_mixed_surface_batch_and_trajectory_copies():
+    jobs = []
+    mats_all = []
+    for j, n in enumerate([5, 9, 13, 14]):
+        g, mats = mixed_circuit(n, 150, seed=j)
+        g = g[(g["op"] != 0xE2) & (g["op"] != 0xE3)]  # matrices differ per job: drop U2/U4
+        jobs.append((n, g))
+    with Batch(jobs, tile_qubits=10, trajectories=3) as b:
+        b.run()
+        b.synchronize()
+        for j, (n, g) in enumerate(jobs):
+            ref = po.state(n)
+            po.run_gates(ref, g, order=b.order(j))
+            for t in range(3):
+                assert np.array_equal(b.download(j, t), ref), (j, t)
