@@ -45,11 +45,12 @@ _SIGS = {
     "orc_qs_apply_noise": [P, U64, U32, I32, D, C.POINTER(OrcRng)],
     "orc_traj_pauli": [U64, U32, U32, U32, D],
     "orc_run_trajectory": [P, U64, P, U32, P, U32, U64, U32, D, D],
+    "orc_run_trajectory_ordered": [P, U64, P, P, U32, P, U32, U64, U32, D, D],
 }
 _RES = {"orc_normalize": I32, "orc_measure": I32, "orc_run_gates": I32,
         "orc_run_gates_ordered": I32, "orc_splitmix64_next": U64, "orc_rng_u64": U64,
         "orc_rng_f64": D, "orc_norm_sqr": D, "orc_qs_apply": I32, "orc_qs_measure": I32,
-        "orc_traj_pauli": U32, "orc_run_trajectory": I32}
+        "orc_traj_pauli": U32, "orc_run_trajectory": I32, "orc_run_trajectory_ordered": I32}
 
 _lib = None
 
@@ -176,10 +177,15 @@ class QS:
 
 
 def run_trajectory(a: np.ndarray, gates: np.ndarray, seed: int, traj: int, p1: float, p2: float,
-                   flags: int = 0, matrices=None) -> None:
+                   flags: int = 0, matrices=None, order=None) -> None:
     g = np.ascontiguousarray(gates)
     m = None if matrices is None else _p(np.ascontiguousarray(matrices, dtype=np.complex128))
-    rc = lib().orc_run_trajectory(_p(a), a.size, _p(g), len(g), m, flags, seed, traj, p1, p2)
+    if order is None:
+        rc = lib().orc_run_trajectory(_p(a), a.size, _p(g), len(g), m, flags, seed, traj, p1, p2)
+    else:
+        o = np.ascontiguousarray(order, dtype=np.uint32)
+        rc = lib().orc_run_trajectory_ordered(_p(a), a.size, _p(g), _p(o), len(g), m, flags,
+                                              seed, traj, p1, p2)
     if rc != 0:
         raise ValueError("oracle rejected a gate")
 

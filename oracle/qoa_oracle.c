@@ -602,3 +602,25 @@ int orc_run_trajectory(double* a, uint64_t len, const qoa_gate* g, uint32_t n,
   }
   return 0;
 }
+
+/* orc_run_trajectory with the gates taken in the order `order` (a planner's
+   executed order); the noise counter is the gate's original index. */
+int orc_run_trajectory_ordered(double* a, uint64_t len, const qoa_gate* g, const uint32_t* order,
+                               uint32_t n, const double* mats, uint32_t flags, uint64_t seed,
+                               uint32_t traj, double p1, double p2) {
+  for (uint32_t k = 0; k < n; ++k) {
+    const qoa_gate* gk = &g[order[k]];
+    if (apply_gate(a, len, gk, mats, flags)) return -1;
+    int ar = gate_arity(gk->op);
+    double p = ar == 1 ? p1 : p2;
+    if (p <= 0.0) continue;
+    uint32_t qs[3] = {gk->q0, gk->q1, gk->q2};
+    for (int j = 0; j < ar; ++j) {
+      uint32_t e = orc_traj_pauli(seed, traj, order[k], qs[j], p);
+      if (e == 1) orc_x(a, len, qs[j]);
+      else if (e == 2) orc_y(a, len, qs[j]);
+      else if (e == 3) orc_z(a, len, qs[j]);
+    }
+  }
+  return 0;
+}

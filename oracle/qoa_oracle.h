@@ -93,6 +93,9 @@ uint32_t orc_traj_pauli(uint64_t seed, uint32_t traj, uint32_t gate_idx, uint32_
 int orc_run_trajectory(double* a, uint64_t len, const qoa_gate* g, uint32_t n,
                        const double* matrices, uint32_t flags, uint64_t seed, uint32_t traj,
                        double p1, double p2);
+int orc_run_trajectory_ordered(double* a, uint64_t len, const qoa_gate* g, const uint32_t* order,
+                               uint32_t n, const double* matrices, uint32_t flags, uint64_t seed,
+                               uint32_t traj, double p1, double p2);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,8 @@ class Batch:
 
     def __init__(self, jobs: Sequence[tuple[int, np.ndarray]], tile_qubits: int = 12,
                  low_qubits: int = 0, flags: int = 0, trajectories: int = 1,
-                 matrices: np.ndarray | None = None):
+                 matrices: np.ndarray | None = None, p1: float = 0.0, p2: float = 0.0,
+                 noise_seed: int | Sequence[int] = 0):
         self._lib = load()
         arrays = [gates_array(g) for _, g in jobs]
         self.gates = np.ascontiguousarray(np.concatenate(arrays)) if arrays else \
@@ -25,7 +26,8 @@ class Batch:
         off = 0
         self.n = []
         for i, ((n, _), g) in enumerate(zip(jobs, arrays)):
-            descs[i] = N.QoaJobDesc(int(n), len(g), off, 0, trajectories, N.QOA_C128, 0.0, 0.0)
+            seed = noise_seed if isinstance(noise_seed, int) else int(noise_seed[i])
+            descs[i] = N.QoaJobDesc(int(n), len(g), off, seed, trajectories, N.QOA_C128, p1, p2)
             off += len(g)
             self.n.append(int(n))
         self.n_gates = [len(g) for g in arrays]

@@ -28,19 +28,25 @@ constexpr int kBatchThreads = 256;              // tiles of up to 12 qubits
 
 struct WorkItem {
   uint64_t state_off;    // first amplitude of the instance's state
+  uint64_t seed;         // trajectory noise key
+  double p1, p2;
   uint32_t ops_begin, nops;
   uint32_t block_begin;  // first block of this item in its round
-  uint8_t k, init, pad0, pad1;
+  uint32_t traj;
+  uint32_t frame_in, frame_out;  // slots of the X frame before / after the pass
+  uint8_t k, init, noise, pad0;
   uint8_t spos[16];
+  uint8_t pad1[4];
 };
-static_assert(sizeof(WorkItem) == 40, "WorkItem layout");
+static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
 
 template <typename I>
 __global__ __launch_bounds__(kBatchThreads) void k_batch_reg(double2* __restrict__ states,
                                                             const WorkItem* __restrict__ items,
                                                             int nitems,
                                                             const TileOp* __restrict__ ops,
-                                                            const double* __restrict__ pool) {
+                                                            const double* __restrict__ pool,
+                                                            uint64_t* __restrict__ frames) {
   extern __shared__ double2 lds[];
   int lo = 0, hi = nitems - 1;
   while (lo < hi) {  // last item whose first block is <= blockIdx.x (uniform)
@@ -59,8 +65,17 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_reg(double2* __restrict
     base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
   }
   const bool act = threadIdx.x < (1u << (it.k - kRegBits));
-  reg_tile<I>(states + it.state_off, base, geo, ops + it.ops_begin, (int)it.nops, pool, lds, act,
-              it.init != 0);
+  NoiseArgs nz;
+  nz.on = it.noise;
+  nz.seed = it.seed;
+  nz.p1 = it.p1;
+  nz.p2 = it.p2;
+  nz.traj = it.traj;
+  nz.frame = (it.noise && !it.init) ? frames[it.frame_in] : 0ULL;
+  const uint64_t fout = reg_tile<I>(states + it.state_off, base, geo, ops + it.ops_begin,
+                                    (int)it.nops, pool, lds, act, it.init != 0, nz);
+  // every tile of the instance computes the same frame; the first one stores it
+  if (it.noise && blockIdx.x == it.block_begin && threadIdx.x == 0) frames[it.frame_out] = fout;
 }
 
 struct Instance {
@@ -92,6 +107,8 @@ struct qoa_batch {
   qoa::TileOp* d_ops;
   double* d_pool;
   std::vector<qoa::Round> rounds;
+  uint64_t* frames;  // 2 X-frame slots per instance (noisy trajectories)
+  bool noisy;
   double alg_bytes;  // algorithmic HBM bytes of one run
   hipEvent_t ev0, ev1;
   float last_ms;
@@ -108,6 +125,7 @@ static void batch_free(qoa_batch* b) {
     if (r.d_items) hipFree(r.d_items);
   if (b->d_ops) hipFree(b->d_ops);
   if (b->d_pool) hipFree(b->d_pool);
+  if (b->frames) hipFree(b->frames);
   if (b->states) hipFree(b->states);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
@@ -138,8 +156,22 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
       return set_error(QOA_ERR_ARG, "job %u: gates beyond the gate array", j);
     if (d.trajectories < 1) return set_error(QOA_ERR_ARG, "job %u: trajectories < 1", j);
     if (d.precision != QOA_C128) return set_error(QOA_ERR_UNSUPPORTED, "job %u: only QOA_C128", j);
-    if (d.p1 != 0.0 || d.p2 != 0.0)
-      return set_error(QOA_ERR_UNSUPPORTED, "job %u: noise needs the trajectory API", j);
+    if (d.p1 < 0.0 || d.p1 > 1.0 || d.p2 < 0.0 || d.p2 > 1.0)
+      return set_error(QOA_ERR_ARG, "job %u: noise probabilities outside [0,1]", j);
+    if (d.p1 > 0.0 || d.p2 > 0.0) {  // trajectories need well-formed gates
+      for (uint64_t i = d.gate_offset; i < d.gate_offset + d.n_gates; ++i) {
+        const qoa_gate& g = gates[i];
+        const bool two = g.op == QOA_OP_CNOT || g.op == QOA_OP_CZ || g.op == QOA_OP_CPHASE ||
+                         g.op == QOA_OP_SWAP || g.op == QOA_OP_U4 || g.op == QOA_OP_CY;
+        const bool three = g.op == QOA_OP_CSWAP || g.op == QOA_OP_TOFFOLI;
+        bool ok = g.q0 < d.n_qubits;
+        if (two || three) ok = ok && g.q1 < d.n_qubits && g.q1 != g.q0;
+        if (three) ok = ok && g.q2 < d.n_qubits && g.q2 != g.q0 && g.q2 != g.q1;
+        if (!ok && g.op != QOA_OP_RESET_ALL)
+          return set_error(QOA_ERR_ARG, "job %u gate %llu: noisy trajectories need distinct "
+                           "in-range operands", j, (unsigned long long)(i - d.gate_offset));
+      }
+    }
   }
   qoa_batch* b = new qoa_batch();
   b->magic = kBatchMagic;
@@ -147,6 +179,8 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
   b->states = nullptr;
   b->d_ops = nullptr;
   b->d_pool = nullptr;
+  b->frames = nullptr;
+  b->noisy = false;
   b->ev0 = b->ev1 = nullptr;
   b->last_ms = 0.f;
   b->jobs.assign(jobs, jobs + n_jobs);
@@ -163,8 +197,10 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
         for (;;) {
           const uint32_t j = next.fetch_add(1);
           if (j >= n_jobs) break;
+          const bool noisy = jobs[j].p1 > 0.0 || jobs[j].p2 > 0.0;
           PlanOptions po{std::min(tile, jobs[j].n_qubits), opts ? opts->low_qubits : 0u,
-                         (opts ? (opts->flags & QOA_CZ_REF_MASKS) : 0u) | kPlanForceReg};
+                         (opts ? (opts->flags & QOA_CZ_REF_MASKS) : 0u) | kPlanForceReg |
+                             (noisy ? kPlanNoise : 0u)};
           rcs[j] = build_plan(jobs[j].n_qubits, gates + jobs[j].gate_offset, jobs[j].n_gates,
                               mats, po, &b->plans[j]);
         }
@@ -218,7 +254,9 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
     batch_free(b);
     return set_error(QOA_ERR_OOM, "batch states need %.2f GiB", 16.0 * off / 1073741824.0);
   }
-  if (hipMalloc(&b->d_ops, sizeof(TileOp) * (all_ops.size() + 1)) != hipSuccess ||
+  for (uint32_t j = 0; j < n_jobs; ++j) b->noisy |= jobs[j].p1 > 0.0 || jobs[j].p2 > 0.0;
+  if (hipMalloc(&b->frames, sizeof(uint64_t) * 2 * b->inst.size()) != hipSuccess ||
+      hipMalloc(&b->d_ops, sizeof(TileOp) * (all_ops.size() + 1)) != hipSuccess ||
       hipMalloc(&b->d_pool, sizeof(double) * (all_pool.size() + 1)) != hipSuccess) {
     (void)hipGetLastError();
     batch_free(b);
@@ -240,12 +278,21 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
     std::vector<WorkItem> items;
     uint64_t blocks = 0;
     Round rd;
-    for (const Instance& in : b->inst) {
+    for (uint32_t ii = 0; ii < b->inst.size(); ++ii) {
+      const Instance& in = b->inst[ii];
       const FusedPlan& P = b->plans[in.job];
       if (r >= P.passes.size()) continue;
       const PassPlan& pp = P.passes[r];
+      const qoa_job_desc& jd = b->jobs[in.job];
       WorkItem w{};
       w.state_off = in.off;
+      w.noise = (jd.p1 > 0.0 || jd.p2 > 0.0) ? 1 : 0;
+      w.seed = jd.noise_seed;
+      w.p1 = jd.p1;
+      w.p2 = jd.p2;
+      w.traj = in.traj;
+      w.frame_in = 2 * ii + (r & 1);
+      w.frame_out = 2 * ii + ((r + 1) & 1);
       w.ops_begin = ops_base[in.job] + pp.op_begin;
       w.nops = pp.op_end - pp.op_begin;
       w.block_begin = (uint32_t)blocks;
@@ -292,7 +339,8 @@ int qoa_batch_run(qoa_batch* b) {
   for (const Round& rd : b->rounds) {
     const size_t lbytes = sizeof(double2) * ((1u << rd.kmax) + (1u << rd.kmax) / 16);
     hipLaunchKernelGGL((k_batch_reg<uint32_t>), dim3(rd.nblocks), dim3(kBatchThreads), lbytes,
-                       b->stream, b->states, rd.d_items, rd.nitems, b->d_ops, b->d_pool);
+                       b->stream, b->states, rd.d_items, rd.nitems, b->d_ops, b->d_pool,
+                       b->frames);
     QOA_HIP_CHECK(hipGetLastError());
   }
   QOA_HIP_CHECK(hipEventRecord(b->ev1, b->stream));
@@ -325,7 +373,20 @@ int qoa_batch_download(qoa_batch* b, uint32_t job, uint32_t traj, double* amps, 
   if (!amps || n_amps != (1ULL << in.n)) return set_error(QOA_ERR_ARG, "output must hold 2^n amplitudes");
   QOA_HIP_CHECK(hipMemcpyAsync(amps, b->states + in.off, sizeof(double2) * n_amps,
                                hipMemcpyDeviceToHost, b->stream));
+  uint64_t frame = 0;
+  const uint32_t np = (uint32_t)b->plans[job].passes.size();
+  const qoa_job_desc& jd = b->jobs[job];
+  if ((jd.p1 > 0.0 || jd.p2 > 0.0) && np > 0)
+    QOA_HIP_CHECK(hipMemcpyAsync(&frame, b->frames + 2 * (b->first_inst[job] + traj) + (np & 1),
+                                 sizeof(uint64_t), hipMemcpyDeviceToHost, b->stream));
   QOA_HIP_CHECK(hipStreamSynchronize(b->stream));
+  if (frame) {  // logical amplitude i is stored at physical index i ^ frame
+    std::vector<double> tmp(amps, amps + 2 * n_amps);
+    for (uint64_t i = 0; i < n_amps; ++i) {
+      amps[2 * i] = tmp[2 * (i ^ frame)];
+      amps[2 * i + 1] = tmp[2 * (i ^ frame) + 1];
+    }
+  }
   return QOA_OK;
 }
 

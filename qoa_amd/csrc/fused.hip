@@ -156,7 +156,7 @@ __global__ __launch_bounds__(512) void k_fused_reg(double2* __restrict__ a,
     const int b = geo.spos[j];
     base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
   }
-  reg_tile<I>(a, base, geo, ops, nops, pool, lds, true, false);
+  reg_tile<I>(a, base, geo, ops, nops, pool, lds, true, false, NoiseArgs{});
 }
 
 // FNV-1a over the bytes that determine a plan.

@@ -35,6 +35,8 @@ struct FOp {
   const double* m4 = nullptr;
   uint32_t gi = 0;
   uint64_t active = 0, all = 0;
+  uint8_t nq = 0, psel = 1;  // trajectory-noise operands (the gate's qubits)
+  uint8_t qphys[3] = {0, 0, 0};
 };
 
 inline void polar(double th, double* out) {  // Complex64::from_polar(1.0, th)
@@ -152,6 +154,17 @@ int canonical(uint32_t n, const qoa_gate& g, const double* mats, uint32_t flags,
   }
   f.active = (f.t >= 0 ? bit(f.t) : 0) | (f.t2 >= 0 ? bit(f.t2) : 0);
   f.all = f.active | f.pmask | (f.rzq >= 0 ? bit(f.rzq) : 0);
+  // noise operands in gate order: one-qubit gates use p1, the others p2
+  switch (g.op) {
+    case QOA_OP_CNOT: case QOA_OP_CZ: case QOA_OP_CPHASE: case QOA_OP_SWAP:
+    case QOA_OP_U4: case QOA_OP_CY: f.nq = 2; break;
+    case QOA_OP_CSWAP: case QOA_OP_TOFFOLI: f.nq = 3; break;
+    default: f.nq = 1; break;
+  }
+  f.psel = f.nq == 1 ? 1 : 2;
+  f.qphys[0] = (uint8_t)q0;
+  f.qphys[1] = (uint8_t)q1;
+  f.qphys[2] = (uint8_t)q2;
   out->push_back(f);
   return 0;
 }
@@ -198,8 +211,21 @@ void set_ins(TileOp* op, uint32_t bits) {
 }
 
 void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<TileOp>* out,
-          std::vector<double>* pool) {
+          std::vector<double>* pool, bool noise) {
   TileOp op{};
+  op.gi = f.gi;
+  if (noise) {
+    op.nq = f.nq;
+    op.psel = f.psel;
+    bool outside = (f.pmask & ~S) != 0 || (f.rzq >= 0 && !(S & bit(f.rzq)));
+    for (int j = 0; j < 3; ++j) {
+      op.qphys[j] = f.qphys[j];
+      if (j < f.nq && !(S & bit(f.qphys[j]))) outside = true;
+    }
+    // with noise, an X error on an outside qubit changes the tile's frame, so
+    // ops that read or hit outside qubits keep their relative order
+    op.pad[0] = outside ? 1 : 0;
+  }
   const uint64_t pin = f.pmask & S, pout = f.pmask & ~S;
   op.lmask = local_bits(pin, spos, k);
   op.gmask = pout;
@@ -230,6 +256,7 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
         o1.gval = bit(f.rzq);
         o1.c[0] = f.c[2];
         o1.c[1] = f.c[3];
+        o0.nq = 0;  // the gate's noise is drawn once, after the second op
         out->push_back(o0);
         out->push_back(o1);
         return;
@@ -265,7 +292,7 @@ struct SegOp {
   uint32_t act, all;
 };
 
-inline SegOp seg_masks(const TileOp& op) {
+inline SegOp seg_masks_local(const TileOp& op) {
   switch (op.kind) {
     case TK_H: case TK_M: case TK_X: case TK_Y:
       return {1u << op.t, (1u << op.t) | op.lmask};
@@ -276,6 +303,12 @@ inline SegOp seg_masks(const TileOp& op) {
     case TK_RZ: return {0u, 1u << op.t};
     default: return {0u, op.lmask};
   }
+}
+
+inline SegOp seg_masks(const TileOp& op) {
+  SegOp m = seg_masks_local(op);
+  if (op.pad[0]) m.all |= 1u << 31;  // ordered against other outside-qubit ops
+  return m;
 }
 
 int seg_sim(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint32_t R,
@@ -473,7 +506,7 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
       const FOp& f = ops[i];
       if ((f.all & blocked) == 0 && (f.active & ~S) == 0) {
         const size_t before = pass_ops.size();
-        emit(f, S, pp.spos, (int)k, &pass_ops, &plan->pool);
+        emit(f, S, pp.spos, (int)k, &pass_ops, &plan->pool, (opt.flags & kPlanNoise) != 0);
         for (size_t e = before; e < pass_ops.size(); ++e) pass_gi.push_back(f.gi);
       } else {
         blocked |= f.all;

@@ -30,6 +30,8 @@ constexpr int kRegAmps = 1 << kRegBits;
 // PlanOptions.flags bit (internal): register mode for every tile of at least
 // kRegBits qubits (the batched kernel runs small whole-state tiles this way).
 constexpr uint32_t kPlanForceReg = 1u << 31;
+// PlanOptions.flags bit (internal): attach trajectory-noise operands to ops.
+constexpr uint32_t kPlanNoise = 1u << 30;
 
 // One op of a pass program. `ins` holds the sorted local bit positions into
 // which a zero is deposited to enumerate the visited indices (nins == 0xFF:
@@ -41,8 +43,16 @@ struct alignas(16) TileOp {
   uint32_t aux;
   uint64_t gmask, gval;  // op applies to a tile iff (tile_base & gmask) == gval
   double c[8];
+  // Trajectory noise after this op (evaluated in every tile, whether or not
+  // the op's condition held there): for each of the nq operand qubits
+  // qphys[j] a Philox draw keyed by (seed, trajectory, gi, qphys[j]) picks
+  // X/Y/Z with probability p1 (psel == 1) or p2 (psel == 2).
+  uint32_t gi;
+  uint8_t nq, psel;
+  uint8_t qphys[3];
+  uint8_t pad[5];
 };
-static_assert(sizeof(TileOp) == 96, "TileOp layout");
+static_assert(sizeof(TileOp) == 112, "TileOp layout");
 
 struct PassPlan {
   uint32_t k;                 // tile qubits

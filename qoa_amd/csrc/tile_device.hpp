@@ -150,17 +150,116 @@ __device__ __forceinline__ void make_layout(uint32_t tid, int k, const uint8_t* 
   }
 }
 
+// Trajectory noise state of one tile pass (on == 0: no noise).
+struct NoiseArgs {
+  uint64_t seed;
+  double p1, p2;
+  uint32_t traj;
+  uint32_t on;
+  uint64_t frame;  // X frame entering the pass: logical index = physical ^ frame
+};
+
+// Philox4x32-10 (Salmon et al., SC'11) with counter (gi, q, traj, const) and
+// key = seed: 0 = no error, else 1 + floor(3 * o2 / 2^32) picks X, Y, Z.
+__device__ __forceinline__ uint32_t traj_pauli(uint64_t seed, uint32_t traj, uint32_t gi,
+                                               uint32_t q, double p) {
+  uint32_t c0 = gi, c1 = q, c2 = traj, c3 = 0x7A3C5E1Fu;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+  }
+  const double u = (double)((((uint64_t)c1 << 32) | c0) >> 11) * 0x1.0p-53;
+  if (!(u < p)) return 0u;
+  return 1u + (uint32_t)(((uint64_t)c2 * 3ULL) >> 32);
+}
+
+template <int J, bool Y>
+__device__ __forceinline__ void reg_flip(double2 (&v)[kR]) {
+#pragma unroll
+  for (int r = 0; r < kR; ++r) {
+    if (r & (1 << J)) continue;
+    const int r1 = r | (1 << J);
+    const double2 x = v[r], y = v[r1];
+    if (Y) {
+      v[r] = cmul(y, make_double2(0.0, -1.0));
+      v[r1] = cmul(x, make_double2(0.0, 1.0));
+    } else {
+      v[r] = y;
+      v[r1] = x;
+    }
+  }
+}
+
+// Pauli `code` (1 X, 2 Y, 3 Z) on local bit b of a register-resident tile,
+// with the arithmetic of apply_{x,y,z}_simd (vectorization.rs:942-999).
+__device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint32_t b,
+                                          const uint8_t* rpos, uint32_t thr_l,
+                                          const uint32_t (&regl)[kR], double2* lds, bool act) {
+  if (code == 3) {
+#pragma unroll
+    for (int r = 0; r < kR; ++r)
+      if (((thr_l | regl[r]) >> b) & 1u) v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
+    return;
+  }
+  int slot = -1;
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j)
+    if (rpos[j] == b) slot = j;
+  if (slot >= 0) {
+    const bool y = code == 2;
+    switch (slot) {
+      case 0: if (y) reg_flip<0, true>(v); else reg_flip<0, false>(v); break;
+      case 1: if (y) reg_flip<1, true>(v); else reg_flip<1, false>(v); break;
+      case 2: if (y) reg_flip<2, true>(v); else reg_flip<2, false>(v); break;
+      default: if (y) reg_flip<3, true>(v); else reg_flip<3, false>(v); break;
+    }
+    return;
+  }
+  // b is a thread bit: exchange partners through LDS
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < kR; ++r) lds[lpad(thr_l | regl[r])] = v[r];
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < kR; ++r) v[r] = lds[lpad((thr_l | regl[r]) ^ (1u << b))];
+  }
+  __syncthreads();
+  if (code == 2) {
+#pragma unroll
+    for (int r = 0; r < kR; ++r)
+      v[r] = (((thr_l | regl[r]) >> b) & 1u) ? cmul(v[r], make_double2(0.0, 1.0))
+                                             : cmul(v[r], make_double2(0.0, -1.0));
+  }
+}
+
 // One register-resident pass over the tile with base `base` (bits of S clear).
 // Threads with act == false (beyond 2^(k-4)) only take part in barriers.
+// With trajectory noise, errors on qubits of the tile are applied in place;
+// an X or Y on a qubit outside the tile flips that bit of the X frame
+// (its phase, a tile constant, is applied at once), and the frame bits that
+// fall inside the tile are applied when the tile is loaded. Returns the frame
+// leaving the pass.
 template <typename I>
-__device__ __forceinline__ void reg_tile(double2* __restrict__ a, I base, const Geo& geo,
-                                         const TileOp* __restrict__ ops, int nops,
-                                         const double* __restrict__ pool, double2* lds, bool act,
-                                         bool init_zero) {
+__device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, const Geo& geo,
+                                             const TileOp* __restrict__ ops, int nops,
+                                             const double* __restrict__ pool, double2* lds,
+                                             bool act, bool init_zero, const NoiseArgs& nz) {
   const int k = geo.k;
   const uint32_t tid = threadIdx.x;
-  // Start layout: the top 4 local bits in registers, the thread index on the
-  // low local bits (HBM runs stay contiguous across lanes).
   uint8_t rpos[kRegBits];
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(k - kRegBits + j);
@@ -177,85 +276,134 @@ __device__ __forceinline__ void reg_tile(double2* __restrict__ a, I base, const 
 #pragma unroll
     for (int r = 0; r < kR; ++r) v[r] = a[base | gthr | scatter<I>(regl[r], geo)];
   }
+  uint64_t frame = nz.on ? nz.frame : 0ULL;
+  if (frame) {
+    uint64_t smask = 0;
+    uint32_t xl = 0;
+    for (int j = 0; j < k; ++j) {
+      smask |= 1ULL << geo.spos[j];
+      if ((frame >> geo.spos[j]) & 1ULL) xl |= 1u << j;
+    }
+    if (xl) {  // frame bits inside the tile: apply those X flips now
+      if (act) {
+#pragma unroll
+        for (int r = 0; r < kR; ++r) lds[lpad(thr_l | regl[r])] = v[r];
+      }
+      __syncthreads();
+      if (act) {
+#pragma unroll
+        for (int r = 0; r < kR; ++r) v[r] = lds[lpad((thr_l | regl[r]) ^ xl)];
+      }
+      __syncthreads();
+      frame &= ~smask;
+    }
+  }
 
   for (int o = 0; o < nops; ++o) {
     const TileOp& op = ops[o];
-    if (((uint64_t)base & op.gmask) != op.gval) continue;  // uniform over the tile
-    switch (op.kind) {
-      case TK_LAYOUT: {
-        if (act) {
+    if ((((uint64_t)base ^ frame) & op.gmask) == op.gval) {  // uniform over the tile
+      switch (op.kind) {
+        case TK_LAYOUT: {
+          if (act) {
 #pragma unroll
-          for (int r = 0; r < kR; ++r) lds[lpad(thr_l | regl[r])] = v[r];
+            for (int r = 0; r < kR; ++r) lds[lpad(thr_l | regl[r])] = v[r];
+          }
+          __syncthreads();
+#pragma unroll
+          for (int j = 0; j < kRegBits; ++j) rpos[j] = op.ins[j];
+          make_layout(tid, k, rpos, thr_l, regl);
+          if (act) {
+#pragma unroll
+            for (int r = 0; r < kR; ++r) v[r] = lds[lpad(thr_l | regl[r])];
+          }
+          __syncthreads();
+        } break;
+        case TK_H: reg_pair_j<TK_H>(v, op, thr_l, regl); break;
+        case TK_M: reg_pair_j<TK_M>(v, op, thr_l, regl); break;
+        case TK_X: reg_pair_j<TK_X>(v, op, thr_l, regl); break;
+        case TK_Y: reg_pair_j<TK_Y>(v, op, thr_l, regl); break;
+        case TK_DNEG: {
+#pragma unroll
+          for (int r = 0; r < kR; ++r)
+            if (((thr_l | regl[r]) & op.lmask) == op.lmask)
+              v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
+        } break;
+        case TK_DPHASE: {
+          const double2 f = make_double2(op.c[0], op.c[1]);
+#pragma unroll
+          for (int r = 0; r < kR; ++r)
+            if (((thr_l | regl[r]) & op.lmask) == op.lmask) v[r] = cmul(v[r], f);
+        } break;
+        case TK_RZ: {
+          const double2 m00 = make_double2(op.c[0], op.c[1]);
+          const double2 m11 = make_double2(op.c[2], op.c[3]);
+#pragma unroll
+          for (int r = 0; r < kR; ++r)
+            v[r] = (((thr_l | regl[r]) >> op.t) & 1u) ? cmul(v[r], m11) : cmul(v[r], m00);
+        } break;
+        case TK_SWAP: {
+          const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
+          const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
+          switch (a0 * 4 + b0) {
+            case 1: reg_swap<0, 1>(v, op, thr_l, regl); break;
+            case 2: reg_swap<0, 2>(v, op, thr_l, regl); break;
+            case 3: reg_swap<0, 3>(v, op, thr_l, regl); break;
+            case 6: reg_swap<1, 2>(v, op, thr_l, regl); break;
+            case 7: reg_swap<1, 3>(v, op, thr_l, regl); break;
+            default: reg_swap<2, 3>(v, op, thr_l, regl); break;
+          }
+        } break;
+        case TK_MAT4: {
+          const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
+          const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
+          const double* u = pool + op.aux;
+          switch (a0 * 4 + b0) {
+            case 1: reg_mat4<0, 1>(v, u); break;
+            case 2: reg_mat4<0, 2>(v, u); break;
+            case 3: reg_mat4<0, 3>(v, u); break;
+            case 6: reg_mat4<1, 2>(v, u); break;
+            case 7: reg_mat4<1, 3>(v, u); break;
+            default: reg_mat4<2, 3>(v, u); break;
+          }
+        } break;
+        default:
+          break;
+      }
+    }
+    if (nz.on && op.nq) {  // trajectory errors after the gate, operand order
+      const double p = op.psel == 1 ? nz.p1 : nz.p2;
+      for (int j = 0; j < op.nq; ++j) {
+        const uint32_t q = op.qphys[j];
+        const uint32_t code = traj_pauli(nz.seed, nz.traj, op.gi, q, p);
+        if (!code) continue;
+        int lb = -1;
+        for (int s = 0; s < k; ++s)
+          if (geo.spos[s] == q) lb = s;
+        if (lb >= 0) {
+          reg_pauli(v, code, (uint32_t)lb, rpos, thr_l, regl, lds, act);
+        } else if (code == 3) {  // Z on an outside qubit: a tile-wide sign
+          if ((((uint64_t)base ^ frame) >> q) & 1ULL) {
+#pragma unroll
+            for (int r = 0; r < kR; ++r) v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
+          }
+        } else {
+          frame ^= 1ULL << q;
+          if (code == 2) {  // Y: phase -i / +i by the new logical bit
+            const double2 f = ((((uint64_t)base ^ frame) >> q) & 1ULL) ? make_double2(0.0, 1.0)
+                                                                        : make_double2(0.0, -1.0);
+#pragma unroll
+            for (int r = 0; r < kR; ++r) v[r] = cmul(v[r], f);
+          }
         }
-        __syncthreads();
-        uint8_t np[kRegBits];
-#pragma unroll
-        for (int j = 0; j < kRegBits; ++j) np[j] = op.ins[j];
-        make_layout(tid, k, np, thr_l, regl);
-        if (act) {
-#pragma unroll
-          for (int r = 0; r < kR; ++r) v[r] = lds[lpad(thr_l | regl[r])];
-        }
-        __syncthreads();
-      } break;
-      case TK_H: reg_pair_j<TK_H>(v, op, thr_l, regl); break;
-      case TK_M: reg_pair_j<TK_M>(v, op, thr_l, regl); break;
-      case TK_X: reg_pair_j<TK_X>(v, op, thr_l, regl); break;
-      case TK_Y: reg_pair_j<TK_Y>(v, op, thr_l, regl); break;
-      case TK_DNEG: {
-#pragma unroll
-        for (int r = 0; r < kR; ++r)
-          if (((thr_l | regl[r]) & op.lmask) == op.lmask)
-            v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
-      } break;
-      case TK_DPHASE: {
-        const double2 f = make_double2(op.c[0], op.c[1]);
-#pragma unroll
-        for (int r = 0; r < kR; ++r)
-          if (((thr_l | regl[r]) & op.lmask) == op.lmask) v[r] = cmul(v[r], f);
-      } break;
-      case TK_RZ: {
-        const double2 m00 = make_double2(op.c[0], op.c[1]), m11 = make_double2(op.c[2], op.c[3]);
-#pragma unroll
-        for (int r = 0; r < kR; ++r)
-          v[r] = (((thr_l | regl[r]) >> op.t) & 1u) ? cmul(v[r], m11) : cmul(v[r], m00);
-      } break;
-      case TK_SWAP: {
-        const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
-        const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
-        const int key = a0 * 4 + b0;
-        switch (key) {
-          case 1: reg_swap<0, 1>(v, op, thr_l, regl); break;
-          case 2: reg_swap<0, 2>(v, op, thr_l, regl); break;
-          case 3: reg_swap<0, 3>(v, op, thr_l, regl); break;
-          case 6: reg_swap<1, 2>(v, op, thr_l, regl); break;
-          case 7: reg_swap<1, 3>(v, op, thr_l, regl); break;
-          default: reg_swap<2, 3>(v, op, thr_l, regl); break;
-        }
-      } break;
-      case TK_MAT4: {
-        const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
-        const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
-        const double* u = pool + op.aux;
-        switch (a0 * 4 + b0) {
-          case 1: reg_mat4<0, 1>(v, u); break;
-          case 2: reg_mat4<0, 2>(v, u); break;
-          case 3: reg_mat4<0, 3>(v, u); break;
-          case 6: reg_mat4<1, 2>(v, u); break;
-          case 7: reg_mat4<1, 3>(v, u); break;
-          default: reg_mat4<2, 3>(v, u); break;
-        }
-      } break;
-      default:
-        break;
+      }
     }
   }
   if (act) {
 #pragma unroll
     for (int r = 0; r < kR; ++r) a[base | gthr | scatter<I>(regl[r], geo)] = v[r];
   }
+  return frame;
 }
-
 
 }  // namespace tile
 }  // namespace qoa

@@ -54,3 +54,30 @@ def test_mixed_surface_batch_and_trajectory_copies():
             po.run_gates(ref, g, order=b.order(j))
             for t in range(3):
                 assert np.array_equal(b.download(j, t), ref), (j, t)
+
+
+@pytest.mark.parametrize("tile", [8, 10, 12])
+def test_noisy_trajectories_match_oracle(tile):
+    """In-kernel Philox noise: each trajectory equals the oracle trajectory
+    (same draws, planned order) bit for bit, including errors on qubits
+    outside a pass's tile (X frame) for the multi-pass jobs."""
+    p1, p2 = 0.05, 0.2
+    jobs = []
+    for j, n in enumerate([5, 9, 12, 14, 16]):
+        _, _, g = W.ansatz_job(300 + j, n_min=n, n_max=n, d_min=6, d_max=6)
+        jobs.append((n, g))
+    seeds = [1000 + j for j in range(len(jobs))]
+    T = 4
+    with Batch(jobs, tile_qubits=tile, trajectories=T, p1=p1, p2=p2, noise_seed=seeds) as b:
+        b.run()
+        b.synchronize()
+        for j, (n, g) in enumerate(jobs):
+            order = b.order(j)
+            outs = []
+            for t in range(T):
+                ref = po.state(n)
+                po.run_trajectory(ref, g, seeds[j], t, p1, p2, order=order)
+                out = b.download(j, t)
+                assert np.array_equal(out, ref), (j, n, t)
+                outs.append(out)
+            assert any(not np.array_equal(outs[0], o) for o in outs[1:]), "noise had no effect"
