@@ -1,22 +1,28 @@
 #!/usr/bin/env python3
 """Gate-application benchmark on one MI355X.
 
-Workload (default): the 28-qubit layered random circuit, depth 40, with CPHASE
-ladders on every 5th layer (3164 gates), complex128 state resident in HBM
-(4 GiB), FP64 arithmetic. One step = the whole circuit applied once.
+Headline (metric/value): the 28-qubit layered random circuit, depth 40, with
+CPHASE ladders on every 5th layer (3164 gates), complex128 state resident in
+HBM (4 GiB), FP64 arithmetic, run with fused register-resident passes. One
+step = the whole circuit applied once. value = amp-updates/s (sum over gates
+of 2^n per second).
 
-Prints one JSON line: amp-updates/s (sum over gates of 2^n per second), the
-roofline of the dominant kernel (algorithmic HBM bytes per launch / average
-launch time, from HIP events on the state's stream) and a CPU baseline (the
-CPU restatement of the reference path on a bounded sample of the same
-circuit).
+Also reported in the same JSON line:
+  roofline      dominant kernel: algorithmic HBM bytes per launch / average
+                launch time (HIP events on the state's stream), vs 8 TB/s
+  cpu_baseline  the CPU restatement of the reference loops on a bounded
+                sample of the same circuit (1 thread)
+  extras        per-gate method on the same circuit; 16-qubit circuit
+                latency; 1024-job batched ansatz (jobs/s); a Poisson serving
+                run (p50/p99 latency, jobs/s)
 
-    python bench.py [--steps K] [--warmup W] [--method fused|per-gate]
+    python bench.py [--steps K] [--warmup W] [--no-extras] [--serve-rates 500,2000]
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -34,10 +40,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, gates: np.ndarray, sample_gates: int, method_label: str) -> dict:
+def cpu_baseline(n: int, gates: np.ndarray, sample_gates: int) -> dict:
     """Times the CPU restatement (oracle, 1 thread) on the first gates of the
-    same circuit and extrapolates per gate (the cost of a sweep does not depend
-    on its position in the circuit)."""
+    same circuit; the cost of a sweep does not depend on its position."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po  # the checker's CPU path, timed as the baseline
 
@@ -50,22 +55,130 @@ def cpu_baseline(n: int, gates: np.ndarray, sample_gates: int, method_label: str
     return {"value": rate, "unit": "amp-updates/s", "cores": 1, "kind": "port",
             "sample": f"first {len(sub)} gates of the {n}-qubit circuit on the CPU "
                       f"restatement of src/vectorization.rs (1 thread, {dt:.1f} s), "
-                      f"extrapolated per gate; host {os.cpu_count()} logical CPUs"}
+                      f"extrapolated per gate; host has {os.cpu_count()} logical CPUs"}
+
+
+def kernel_table(sv) -> dict:
+    kinds = {}
+    for kind in ("sweep_dense", "sweep_diag", "sweep_perm", "fused_pass", "reduce", "other"):
+        ms, cnt, b = sv.timing_read(kind)
+        if cnt:
+            kinds[kind] = {"ms": ms, "launches": cnt, "bytes": b,
+                           "gbps": b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
+                           "avg_us": ms * 1e3 / cnt}
+    return kinds
+
+
+def run_circuit(sv, gates, method: str, steps: int, warmup: int, tag: str):
+    def step():
+        if method == "per-gate":
+            sv.run_gates(gates)
+        elif method == "fused-lds":
+            sv.run_fused(gates, flags=2)
+        else:
+            sv.run_fused(gates)
+
+    sv.apply_reset_all()
+    for w in range(warmup):
+        step()
+        sv.synchronize()
+        log(f"{tag}: warmup {w + 1}/{warmup} done")
+    sv.timing_enable(True)
+    sv.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step()
+        log(f"{tag}: step {k + 1}/{steps} enqueued")
+    sv.synchronize()
+    elapsed = time.perf_counter() - t0
+    kinds = kernel_table(sv)
+    sv.timing_enable(False)
+    return elapsed, kinds
+
+
+def extra_c1(args) -> dict:
+    from qoa_amd import StateVector
+    from qoa_amd import workloads as W
+
+    n = 16
+    g = W.layered_random(n, 20, seed=args.seed)
+    out = {"workload": "16-qubit layered random circuit, depth 20 (470 gates)"}
+    with StateVector(n) as sv:
+        for method in ("per-gate", "fused"):
+            el, _ = run_circuit(sv, g, method, 20, 2, f"c1 {method}")
+            out[f"{method}_us_per_circuit"] = el / 20 * 1e6
+            out[f"{method}_circuits_per_s"] = 20 / el
+    return out
+
+
+def extra_c4(args) -> dict:
+    from qoa_amd import workloads as W
+    from qoa_amd.batch import Batch
+
+    t0 = time.perf_counter()
+    jobs = [W.ansatz_job(args.seed + j) for j in range(1024)]
+    gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = Batch([(n, g) for n, _, g in jobs], tile_qubits=12)
+    plan_s = time.perf_counter() - t0
+    info = b.info()
+    b.run()
+    b.synchronize()  # warm-up
+    times = []
+    for _ in range(3):
+        b.run()
+        times.append(b.synchronize())
+    ms = min(times)
+    b.close()
+    amp_updates = float(sum(len(g) * (1 << n) for n, _, g in jobs))
+    return {"workload": "1024 hardware-efficient-ansatz circuits, n~U{8..24}, depth~U{10..60}, "
+                        "FP64, all states resident", "jobs": 1024,
+            "state_gib": info["total_amps"] * 16 / 2**30, "rounds": info["rounds"],
+            "ms_per_batch": ms, "jobs_per_s": 1024 / (ms * 1e-3),
+            "amp_updates_per_s": amp_updates / (ms * 1e-3),
+            "hbm_gbps": info["alg_bytes"] / (ms * 1e-3) / 1e9,
+            "gen_s": gen, "plan_s": plan_s}
+
+
+def extra_c5(args, rate: float, n_jobs: int) -> dict:
+    from qoa_amd import workloads as W
+    from qoa_amd.server import Server
+
+    rng = np.random.default_rng(args.seed)
+    jobs = [W.serving_job(args.seed * 7 + j) for j in range(n_jobs)]
+    gaps = rng.exponential(1.0 / rate, size=n_jobs)
+    with Server(budget_bytes=int(200 * 2**30)) as srv:
+        t_start = srv.now() + 0.2
+        arrivals = t_start + np.cumsum(gaps)
+        ids = [srv.submit(n, g, trajectories=64, p1=1e-3, p2=1e-2, noise_seed=args.seed + j,
+                          arrival=float(arrivals[j]))
+               for j, (n, _, g) in enumerate(jobs)]
+        res = [srv.wait(i, timeout=600) for i in ids]
+        stats = srv.stats()
+    ok = [r for r in res if r.status == 0]
+    lat = np.array([r.latency for r in ok])
+    span = max(r.t_done for r in ok) - t_start
+    return {"rate_jobs_per_s": rate, "jobs": n_jobs, "completed": len(ok),
+            "rejected": stats["rejected"], "jobs_per_s": len(ok) / span,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "steps": stats["steps"], "arena_peak_gib": stats["arena_peak"] / 2**30}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--method", default="auto", choices=["auto", "fused", "per-gate"])
+    ap.add_argument("--method", default="fused", choices=["fused", "fused-lds", "per-gate"])
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=20241203)
-    ap.add_argument("--cpu-sample-gates", type=int, default=12)
+    ap.add_argument("--cpu-sample-gates", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--compare", action="store_true",
-                    help="also time the other method and report it")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--serve-rates", default="1000")
+    ap.add_argument("--serve-jobs", type=int, default=1500)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -83,74 +196,50 @@ def main() -> int:
     amp_updates = W.count_gate_amp_updates(n, gates)
     log(f"workload: {n} qubits, depth {args.depth}, {len(gates)} gates")
 
-    method = args.method
-    if method == "auto":
-        method = "fused"
     sv = StateVector(n)
-
-    def step(m: str) -> None:
-        if m == "fused":
-            sv.run_fused(gates)
-        else:
-            sv.run_gates(gates)
-
-    try:
-        step(method)
-    except Exception as e:  # fused path unavailable in this build
-        log(f"{method} unavailable ({e}); using per-gate")
-        method = "per-gate"
-    sv.apply_reset_all()
-    for w in range(args.warmup):
-        step(method)
-        sv.synchronize()
-        log(f"warmup {w + 1}/{args.warmup} done")
-
-    sv.timing_enable(True)
-    sv.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(method)
-        log(f"step {k + 1}/{args.steps} enqueued")
-    sv.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    kinds = {}
-    for kind in ("sweep_dense", "sweep_diag", "sweep_perm", "fused_pass", "reduce", "other"):
-        ms, cnt, b = sv.timing_read(kind)
-        if cnt:
-            kinds[kind] = {"ms": ms, "launches": cnt, "bytes": b,
-                           "gbps": b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
-                           "avg_us": ms * 1e3 / cnt}
-    sv.timing_enable(False)
+    elapsed, kinds = run_circuit(sv, gates, args.method, args.steps, args.warmup, "c2")
     dom = max(kinds.items(), key=lambda kv: kv[1]["ms"])
     tot_ms = sum(v["ms"] for v in kinds.values())
     tot_b = sum(v["bytes"] for v in kinds.values())
-    ach = dom[1]["gbps"]
     value = amp_updates * args.steps / elapsed
     norm = sv.norm_sqr()
-    log(f"{method}: {elapsed / args.steps * 1e3:.1f} ms/step, {value:.3e} amp-updates/s, "
-        f"dominant {dom[0]} {ach:.0f} GB/s, norm^2 {norm:.15f}")
+    log(f"{args.method}: {elapsed / args.steps * 1e3:.1f} ms/step, {value:.3e} amp-updates/s, "
+        f"dominant {dom[0]} {dom[1]['gbps']:.0f} GB/s, norm^2 {norm:.15f}")
 
-    other = None
-    if args.compare:
-        om = "per-gate" if method == "fused" else "fused"
-        try:
-            sv.synchronize()
-            ta = time.perf_counter()
-            step(om)
-            sv.synchronize()
-            other = {"method": om, "ms_per_step": (time.perf_counter() - ta) * 1e3,
-                     "amp_updates_per_s": amp_updates / (time.perf_counter() - ta)}
-        except Exception as e:
-            other = {"method": om, "error": str(e)}
-
-    name = ctypes_name(lib)
+    extras = {}
+    if not args.no_extras:
+        for other in ("per-gate", "fused-lds"):
+            if other == args.method:
+                continue
+            el, kk = run_circuit(sv, gates, other, 1, 0, f"c2 {other}")
+            extras[f"c2_{other}"] = {"ms_per_step": el * 1e3,
+                                     "amp_updates_per_s": amp_updates / el,
+                                     "kernels": kk}
+    name = device_name(lib)
     sv.close()
+
+    if not args.no_extras:
+        for label, fn in (("c1", lambda: extra_c1(args)), ("c4_batch", lambda: extra_c4(args))):
+            try:
+                extras[label] = fn()
+                log(f"{label}: {json.dumps(extras[label])[:200]}")
+            except Exception as e:  # noqa: BLE001
+                extras[label] = {"error": str(e)}
+                log(f"{label} failed: {e}")
+        serve = []
+        for r in [float(x) for x in args.serve_rates.split(",") if x]:
+            try:
+                serve.append(extra_c5(args, r, args.serve_jobs))
+                log(f"c5 serving: {serve[-1]}")
+            except Exception as e:  # noqa: BLE001
+                serve.append({"rate_jobs_per_s": r, "error": str(e)})
+                log(f"c5 serving failed: {e}")
+        extras["c5_serving"] = serve
 
     cpu = None
     if not args.no_cpu:
         log("CPU baseline sample ...")
-        cpu = cpu_baseline(n, gates, args.cpu_sample_gates, method)
+        cpu = cpu_baseline(n, gates, args.cpu_sample_gates)
         log(f"CPU baseline {cpu['value']:.3e} amp-updates/s")
 
     out = {
@@ -169,31 +258,30 @@ def main() -> int:
         "config": {"workload": f"{n}-qubit layered random circuit, depth {args.depth}, CPHASE "
                                f"ladders k=1..8 every 5th layer ({len(gates)} gates), complex128 "
                                "state resident in HBM", "n_qubits": n, "depth": args.depth,
-                   "gates": len(gates), "seed": args.seed, "method": method},
-        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom[0],
-                     "avg_launch_us": dom[1]["avg_us"],
+                   "gates": len(gates), "seed": args.seed, "method": args.method},
+        "roofline": {"bound": "hbm", "achieved": dom[1]["gbps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom[1]["gbps"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": dom[0], "avg_launch_us": dom[1]["avg_us"],
                      "bytes_per_launch": dom[1]["bytes"] / dom[1]["launches"]},
         "hbm_gbps_all_kernels": tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms else None,
         "kernels": kinds,
         "cpu_baseline": cpu,
+        "extras": extras,
         "device": name,
         "final_norm_sqr": norm,
     }
-    if other:
-        out["other_method"] = other
     print(json.dumps(out), flush=True)
     return 0
 
 
-def ctypes_name(lib) -> str:
+def device_name(lib) -> str:
     import ctypes as C
 
     buf = C.create_string_buffer(256)
     hbm = C.c_uint64()
     cus = C.c_int()
     if lib.qoa_device_info(buf, 256, C.byref(hbm), C.byref(cus)) == 0:
-        return f"{buf.value.decode()} {hbm.value / 2**30:.0f} GiB {cus.value} CUs"
+        return f"{buf.value.decode().strip()} {hbm.value / 2**30:.0f} GiB {cus.value} CUs"
     return "unknown"
 
 

@@ -181,6 +181,12 @@ int qoa_measure(qoa_state* s, uint32_t q, double r, int* outcome, double* p0,
                 double* p1);
 /* |a_i|^2 for every i into out[0 .. n_amps). */
 int qoa_probabilities(qoa_state* s, double* out, uint64_t n_amps);
+/* Draws `count` Z-basis outcomes from |a_i|^2 (get_probabilities,
+   quantum_state.rs:128-133) by inverse CDF on the device. Uniform s is
+   Philox4x32-10(counter {0xFFFFFFFF, s, 0, 0x5A3E1D2C}, key seed); the CDF is
+   summed in a fixed order (chunks of 4096, runs of 16), so results are
+   reproducible and independent of timing. The state is not collapsed. */
+int qoa_sample(qoa_state* s, uint64_t seed, uint32_t count, uint64_t* out);
 /* One depolarizing event set: for every qubit q with code[q] in {1,2,3}
    apply X, Y or Z on q (0 = none), in qubit order, as one fused sweep
    (apply_depolarizing_noise, quantum_state.rs:872-885). */
@@ -209,6 +215,16 @@ int qoa_run_fused(qoa_state* s, const qoa_gate* gates, uint32_t n_gates,
    order[0..n_gates) and the pass boundary count into *n_passes. */
 int qoa_plan_fused(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
                    const qoa_fuse_opts* opts, uint32_t* order, uint32_t* n_passes);
+
+/* Introspection of a fused plan (host only): copies the op program (records
+   of qoa_tile_op_size() bytes) and, per pass, the tile qubit positions
+   (16 bytes), the op range and the kernel mode. Returns the counts; pass NULL
+   buffers to query sizes. */
+int qoa_tile_op_size(void);
+int qoa_plan_program(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
+                     const double* matrices, const qoa_fuse_opts* opts, void* ops_out,
+                     uint32_t max_ops, uint8_t* pass_spos_out, uint32_t* pass_range_out,
+                     uint32_t max_passes, uint32_t* n_ops, uint32_t* n_passes, uint32_t* tile_k);
 
 /* Kernel timing: when enabled, every kernel launched on the handle's stream
    is bracketed by HIP events; qoa_timing_read returns per-kind totals. */
@@ -268,6 +284,50 @@ int qoa_batch_download(qoa_batch* b, uint32_t job, uint32_t traj, double* amps,
 /* Executed gate order of a job (for bit-exact checks against the oracle). */
 int qoa_batch_order(qoa_batch* b, uint32_t job, uint32_t* order, uint32_t n_order);
 int qoa_batch_destroy(qoa_batch* b);
+
+/* ---- serving: continuous batching of arriving jobs ------------------------
+   An engine thread plans arrived jobs, admits them in arrival order while
+   their footprint (trajectories x 2^n x 16 B plus scratch) fits the arena,
+   and launches one step at a time: one kernel that runs the next pass of
+   every admitted job. Each trajectory ends with one Z-basis sample
+   (Philox key noise_seed ^ 0x53414D504C45, fixed-order CDF as qoa_sample).
+   A job's result does not depend on the other jobs in flight. */
+typedef struct qoa_server qoa_server;
+typedef struct qoa_server_opts {
+  uint64_t budget_bytes;    /* HBM arena for job states (0: 80% of free memory) */
+  uint32_t tile_qubits;     /* 8..12 (0: 12)                                   */
+  uint32_t planner_threads; /* host planning threads (0: 4)                    */
+  uint32_t max_items;       /* work items per step (0: 65536)                  */
+  uint32_t max_ops;         /* op slots for all resident jobs (0: 2^21)         */
+  uint32_t max_jobs;        /* resident jobs (0: 4096)                          */
+  uint32_t flags;           /* QOA_CZ_REF_MASKS                                 */
+} qoa_server_opts;
+typedef struct qoa_job_result {
+  uint64_t job_id;
+  int32_t status;           /* QOA_OK, QOA_ERR_ADMISSION, ...                  */
+  uint32_t n_qubits, trajectories, n_passes, n_samples;
+  double t_arrival, t_admit, t_done;  /* server clock, seconds               */
+} qoa_job_result;
+typedef struct qoa_server_stats {
+  uint64_t submitted, completed, rejected, steps;
+  uint64_t arena_bytes, arena_peak;
+  int32_t fatal;
+} qoa_server_stats;
+int qoa_server_create(const qoa_server_opts* opts, qoa_server** out);
+/* Copies the job's gates (job->gate_offset is ignored); the job becomes
+   eligible at arrival_s on the server clock (qoa_server_now). A job that can
+   never fit the arena is rejected with QOA_ERR_ADMISSION. */
+int qoa_server_submit(qoa_server* s, const qoa_job_desc* job, const qoa_gate* gates,
+                      const double* matrices, uint32_t n_matrices, double arrival_s,
+                      uint64_t* job_id);
+double qoa_server_now(qoa_server* s);
+/* Blocks until the job finished (or timeout); copies up to max_samples
+   samples (one per trajectory, logical basis indices). Returns the status. */
+int qoa_server_wait(qoa_server* s, uint64_t job_id, double timeout_s, qoa_job_result* res,
+                    uint64_t* samples, uint32_t max_samples);
+int qoa_server_drain(qoa_server* s, double timeout_s);
+int qoa_server_get_stats(qoa_server* s, qoa_server_stats* st);
+int qoa_server_destroy(qoa_server* s);
 
 #ifdef __cplusplus
 }

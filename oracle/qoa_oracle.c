@@ -12,6 +12,7 @@
  * Loops run over every basis index in increasing order with the same
  * predicates as the reference x86_64 module (src/vectorization.rs:904-1261).
  */
+#define _GNU_SOURCE
 #include "qoa_oracle.h"
 
 #include <math.h>
@@ -21,6 +22,12 @@
 #define IM(a, i) (a)[2 * (i) + 1]
 
 static inline uint64_t bit_of(uint64_t q) { return q < 64 ? (1ULL << q) : 0ULL; }
+
+/* cos x and sin x from one glibc sincos call. The reference computes them
+   side by side (Complex64::from_polar, the rx/ry set-up at
+   src/vectorization.rs:529-535), which LLVM lowers to sincos on linux-gnu;
+   sincos and separate cos/sin can differ in the last bit. */
+static inline void cis(double x, double* c, double* s) { sincos(x, s, c); }
 
 /* c = x * y (num-complex Mul). */
 static inline void cmul(double xr, double xi, double yr, double yi, double* cr, double* ci) {
@@ -104,20 +111,24 @@ void orc_phase_factor(double* a, uint64_t len, uint64_t mask_bit, double pf_re, 
 
 /* apply_t_vectorized, src/vectorization.rs:217-218: from_polar(1, pi/4). */
 void orc_t(double* a, uint64_t len, uint64_t mask_bit) {
-  double th = M_PI / 4.0;
-  orc_phase_factor(a, len, mask_bit, 1.0 * cos(th), 1.0 * sin(th));
+  double c, s;
+  cis(M_PI / 4.0, &c, &s);
+  orc_phase_factor(a, len, mask_bit, 1.0 * c, 1.0 * s);
 }
 
 /* apply_s_vectorized, src/vectorization.rs:262-263: from_polar(1, pi/2) =
    (6.123e-17, 1.0), not exactly i. */
 void orc_s(double* a, uint64_t len, uint64_t mask_bit) {
-  double th = M_PI / 2.0;
-  orc_phase_factor(a, len, mask_bit, 1.0 * cos(th), 1.0 * sin(th));
+  double c, s;
+  cis(M_PI / 2.0, &c, &s);
+  orc_phase_factor(a, len, mask_bit, 1.0 * c, 1.0 * s);
 }
 
 /* apply_phaseshift_vectorized, src/vectorization.rs:307-308. */
 void orc_phaseshift(double* a, uint64_t len, uint64_t mask_bit, double angle) {
-  orc_phase_factor(a, len, mask_bit, 1.0 * cos(angle), 1.0 * sin(angle));
+  double c, s;
+  cis(angle, &c, &s);
+  orc_phase_factor(a, len, mask_bit, 1.0 * c, 1.0 * s);
 }
 
 /* apply_reset_simd, src/vectorization.rs:1020-1056. */
@@ -197,7 +208,8 @@ void orc_matrix2(double* a, uint64_t len, uint64_t mask_bit, const double* m) {
 
 /* apply_rx_vectorized, src/vectorization.rs:528-535. */
 void orc_rx(double* a, uint64_t len, uint64_t mask_bit, double angle) {
-  double c = cos(angle / 2.0), s = sin(angle / 2.0);
+  double c, s;
+  cis(angle / 2.0, &c, &s);
   double m[8] = {c, 0.0, 0.0, -s, 0.0, -s, c, 0.0};
   orc_matrix2(a, len, mask_bit, m);
 }
@@ -205,7 +217,8 @@ void orc_rx(double* a, uint64_t len, uint64_t mask_bit, double angle) {
 /* apply_ry_vectorized, src/vectorization.rs:588-595 (apply_ry_simd calls the
    rx loop, :1088-1100). */
 void orc_ry(double* a, uint64_t len, uint64_t mask_bit, double angle) {
-  double c = cos(angle / 2.0), s = sin(angle / 2.0);
+  double c, s;
+  cis(angle / 2.0, &c, &s);
   double m[8] = {c, 0.0, -s, 0.0, s, 0.0, c, 0.0};
   orc_matrix2(a, len, mask_bit, m);
 }
@@ -214,8 +227,10 @@ void orc_ry(double* a, uint64_t len, uint64_t mask_bit, double angle) {
    a *= m00 = from_polar(1,-t/2) if bit clear, else a *= m11 = from_polar(1,t/2).
    With mask_bit >= n every index takes m00 (a global phase). */
 void orc_rz(double* a, uint64_t len, uint64_t mask_bit, double angle) {
-  double m00r = 1.0 * cos(-angle / 2.0), m00i = 1.0 * sin(-angle / 2.0);
-  double m11r = 1.0 * cos(angle / 2.0), m11i = 1.0 * sin(angle / 2.0);
+  double c0, s0, c1, s1;
+  cis(-angle / 2.0, &c0, &s0);
+  cis(angle / 2.0, &c1, &s1);
+  double m00r = 1.0 * c0, m00i = 1.0 * s0, m11r = 1.0 * c1, m11i = 1.0 * s1;
   uint64_t bit = bit_of(mask_bit);
   for (uint64_t i = 0; i < len; ++i) {
     double xr = RE(a, i), xi = IM(a, i);
@@ -258,7 +273,9 @@ void orc_cz(double* a, uint64_t len, uint64_t cmask, uint64_t tmask) {
    788-794, 1235-1250 (MASKS; phase = from_polar(1, angle)). */
 void orc_cphase(double* a, uint64_t len, uint64_t cmask, uint64_t tmask, double angle) {
   uint64_t m = cmask | tmask;
-  double pr = 1.0 * cos(angle), pi = 1.0 * sin(angle);
+  double c, s;
+  cis(angle, &c, &s);
+  double pr = 1.0 * c, pi = 1.0 * s;
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & m) == m) {
       double xr = RE(a, i), xi = IM(a, i);
@@ -623,4 +640,67 @@ int orc_run_trajectory_ordered(double* a, uint64_t len, const qoa_gate* g, const
     }
   }
   return 0;
+}
+
+/* ---- sampling ---------------------------------------------------------------
+   The measurement distribution |a_i|^2 (get_probabilities,
+   src/runtime/quantum_state.rs:128-133) sampled by inverse CDF with sums in a
+   fixed order: chunks of C = min(len, 4096), runs of C / min(256, C)
+   amplitudes, every sum left to right. Returns the first index whose running
+   sum exceeds u * total (or the last non-zero one if rounding leaves none). */
+double orc_sample_uniform(uint64_t seed, uint32_t traj, uint32_t s) {
+  uint32_t c[4] = {0xFFFFFFFFu, s, traj, 0x5A3E1D2Cu};
+  uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t o[4];
+  orc_philox4x32_10(c, k, o);
+  return (double)((((uint64_t)o[1] << 32) | o[0]) >> 11) * 0x1.0p-53;
+}
+
+static double run_sum(const double* a, uint64_t start, uint64_t cnt) {
+  double s = 0.0;
+  for (uint64_t j = 0; j < cnt; ++j)
+    s += RE(a, start + j) * RE(a, start + j) + IM(a, start + j) * IM(a, start + j);
+  return s;
+}
+
+uint64_t orc_sample(const double* a, uint64_t len, double u) {
+  const uint64_t C = len < 4096 ? len : 4096, nch = len / C;
+  const uint64_t runs = C < 256 ? C : 256, per = C / runs;
+  double total = 0.0;
+  for (uint64_t c = 0; c < nch; ++c) {
+    double cs = 0.0;
+    for (uint64_t r = 0; r < runs; ++r) cs += run_sum(a, c * C + r * per, per);
+    total += cs;
+  }
+  const double target = u * total;
+  double acc = 0.0, acc_last = 0.0;
+  int64_t chosen = -1, lastnz = -1;
+  for (uint64_t c = 0; c < nch; ++c) {
+    double cs = 0.0;
+    for (uint64_t r = 0; r < runs; ++r) cs += run_sum(a, c * C + r * per, per);
+    if (cs > 0.0) { lastnz = (int64_t)c; acc_last = acc; }
+    if (acc + cs > target) { chosen = (int64_t)c; break; }
+    acc += cs;
+  }
+  if (chosen < 0) { chosen = lastnz < 0 ? 0 : lastnz; acc = lastnz < 0 ? 0.0 : acc_last; }
+  const double acc_chunk = acc;
+  int64_t rsel = -1, rlast = -1;
+  acc_last = acc;
+  for (uint64_t r = 0; r < runs; ++r) {
+    const double ps = run_sum(a, (uint64_t)chosen * C + r * per, per);
+    if (ps > 0.0) { rlast = (int64_t)r; acc_last = acc; }
+    if (acc + ps > target) { rsel = (int64_t)r; break; }
+    acc += ps;
+  }
+  if (rsel < 0) { rsel = rlast < 0 ? 0 : rlast; acc = rlast < 0 ? acc_chunk : acc_last; }
+  const uint64_t base = (uint64_t)chosen * C + (uint64_t)rsel * per;
+  int64_t pick = -1, jlast = -1;
+  for (uint64_t j = 0; j < per; ++j) {
+    const double p = RE(a, base + j) * RE(a, base + j) + IM(a, base + j) * IM(a, base + j);
+    if (p > 0.0) jlast = (int64_t)j;
+    if (acc + p > target) { pick = (int64_t)j; break; }
+    acc += p;
+  }
+  if (pick < 0) pick = jlast < 0 ? 0 : jlast;
+  return base + (uint64_t)pick;
 }

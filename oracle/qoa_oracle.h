@@ -97,6 +97,10 @@ int orc_run_trajectory_ordered(double* a, uint64_t len, const qoa_gate* g, const
                                uint32_t n, const double* matrices, uint32_t flags, uint64_t seed,
                                uint32_t traj, double p1, double p2);
 
+/* ---- Z-basis sampling with the backend's fixed summation order ---- */
+double orc_sample_uniform(uint64_t seed, uint32_t traj, uint32_t s);
+uint64_t orc_sample(const double* a, uint64_t len, double u);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,9 @@ _SIGS = {
     "qoa_run_gates": ([P, P, U32, P, U32], I32),
     "qoa_run_fused": ([P, P, U32, P, P], I32),
     "qoa_plan_fused": ([U32, P, U32, P, P, C.POINTER(U32)], I32),
+    "qoa_tile_op_size": ([], I32),
+    "qoa_plan_program": ([U32, P, U32, P, P, P, U32, P, P, U32, C.POINTER(U32), C.POINTER(U32),
+                          C.POINTER(U32)], I32),
     "qoa_timing_enable": ([P, I32], I32),
     "qoa_timing_read": ([P, C.c_char_p, DP, C.POINTER(U64), DP], I32),
     "qoa_qs_set_noise": ([P, I32, D], I32),
@@ -171,3 +174,37 @@ _SIGS.update({
     "qoa_batch_order": ([P, U32, P, U32], I32),
     "qoa_batch_destroy": ([P], I32),
 })
+
+
+class QoaServerOpts(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("tile_qubits", C.c_uint32),
+                ("planner_threads", C.c_uint32), ("max_items", C.c_uint32),
+                ("max_ops", C.c_uint32), ("max_jobs", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class QoaJobResult(C.Structure):
+    _fields_ = [("job_id", C.c_uint64), ("status", C.c_int32), ("n_qubits", C.c_uint32),
+                ("trajectories", C.c_uint32), ("n_passes", C.c_uint32),
+                ("n_samples", C.c_uint32), ("t_arrival", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class QoaServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("steps", C.c_uint64), ("arena_bytes", C.c_uint64), ("arena_peak", C.c_uint64),
+                ("fatal", C.c_int32)]
+
+
+P, U32, U64, D, I32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.c_int
+_SIGS.update({
+    "qoa_server_create": ([C.POINTER(QoaServerOpts), C.POINTER(P)], I32),
+    "qoa_server_submit": ([P, C.POINTER(QoaJobDesc), P, P, U32, D, C.POINTER(U64)], I32),
+    "qoa_server_now": ([P], D),
+    "qoa_server_wait": ([P, U64, D, C.POINTER(QoaJobResult), P, U32], I32),
+    "qoa_server_drain": ([P, D], I32),
+    "qoa_server_get_stats": ([P, C.POINTER(QoaServerStats)], I32),
+    "qoa_server_destroy": ([P], I32),
+    "qoa_sample": ([P, U64, U32, P], I32),
+})
+
+

@@ -128,7 +128,15 @@ int timing_collect(qoa_state* s) {
 }
 
 // ---- gate coefficients (host libm, like the reference) -------------------------
-static Cx from_polar(double r, double th) { return Cx{r * std::cos(th), r * std::sin(th)}; }
+// cos/sin pairs come from one sincos call, as rustc/LLVM lowers the
+// reference's side-by-side .cos()/.sin() on linux-gnu (sincos and separate
+// cos/sin can differ in the last bit).
+static inline void cis(double x, double* c, double* s) { ::sincos(x, s, c); }
+static Cx from_polar(double r, double th) {
+  double c, s;
+  cis(th, &c, &s);
+  return Cx{r * c, r * s};
+}
 
 static inline uint64_t bit_or_zero(uint32_t q, uint32_t n) { return q < n ? (1ULL << q) : 0ULL; }
 
@@ -469,7 +477,8 @@ int qoa_apply_controlled_swap(qoa_state* s, uint32_t c, uint32_t t1, uint32_t t2
 int qoa_apply_rx(qoa_state* s, uint32_t q, double angle) {
   CHECK_STATE(s); CHECK_C128(s); CHECK_Q64(q);
   if (q >= s->n) return QOA_OK;
-  const double c = std::cos(angle / 2.0), sn = std::sin(angle / 2.0);  // vectorization.rs:529-535
+  double c, sn;
+  cis(angle / 2.0, &c, &sn);  // vectorization.rs:529-535
   const Cx m[4] = {{c, 0.0}, {0.0, -sn}, {0.0, -sn}, {c, 0.0}};
   return launch_matrix2(s, m, q, 0);
 }
@@ -477,7 +486,8 @@ int qoa_apply_rx(qoa_state* s, uint32_t q, double angle) {
 int qoa_apply_ry(qoa_state* s, uint32_t q, double angle) {
   CHECK_STATE(s); CHECK_C128(s); CHECK_Q64(q);
   if (q >= s->n) return QOA_OK;
-  const double c = std::cos(angle / 2.0), sn = std::sin(angle / 2.0);  // vectorization.rs:589-595
+  double c, sn;
+  cis(angle / 2.0, &c, &sn);  // vectorization.rs:589-595
   const Cx m[4] = {{c, 0.0}, {-sn, 0.0}, {sn, 0.0}, {c, 0.0}};
   return launch_matrix2(s, m, q, 0);
 }

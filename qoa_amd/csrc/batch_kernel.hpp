@@ -1,0 +1,71 @@
+// batch_kernel.hpp — the work-item pass kernel shared by batches and the server.
+//
+// One launch runs one pass for many states: block b finds its work item (the
+// last item whose first block is <= b) and runs a register-resident pass on
+// tile b - item.block_begin of that item's state. Items carry their own
+// trajectory-noise key and X-frame slots.
+#pragma once
+#include "planner.hpp"
+#include "qoa_internal.hpp"
+#include "tile_device.hpp"
+
+namespace qoa {
+namespace tile {
+
+constexpr uint32_t kBatchMagic = 0x51424154u;  // "QBAT"
+constexpr int kBatchThreads = 256;              // tiles of up to 12 qubits
+
+struct WorkItem {
+  uint64_t state_off;    // first amplitude of the instance's state
+  uint64_t seed;         // trajectory noise key
+  double p1, p2;
+  uint32_t ops_begin, nops;
+  uint32_t block_begin;  // first block of this item in its round
+  uint32_t traj;
+  uint32_t frame_in, frame_out;  // slots of the X frame before / after the pass
+  uint8_t k, init, noise, pad0;
+  uint8_t spos[16];
+  uint8_t pad1[4];
+};
+static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
+
+template <typename I>
+__global__ __launch_bounds__(kBatchThreads) void k_batch_reg(double2* __restrict__ states,
+                                                            const WorkItem* __restrict__ items,
+                                                            int nitems,
+                                                            const TileOp* __restrict__ ops,
+                                                            const double* __restrict__ pool,
+                                                            uint64_t* __restrict__ frames) {
+  extern __shared__ double2 lds[];
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) {  // last item whose first block is <= blockIdx.x (uniform)
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block_begin <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const WorkItem& it = items[lo];
+  Geo geo;
+  geo.k = it.k;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) geo.spos[j] = it.spos[j];
+  I base = (I)(blockIdx.x - it.block_begin);
+  for (int j = 0; j < geo.k; ++j) {
+    const int b = geo.spos[j];
+    base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
+  }
+  const bool act = threadIdx.x < (1u << (it.k - kRegBits));
+  NoiseArgs nz;
+  nz.on = it.noise;
+  nz.seed = it.seed;
+  nz.p1 = it.p1;
+  nz.p2 = it.p2;
+  nz.traj = it.traj;
+  nz.frame = (it.noise && !it.init) ? frames[it.frame_in] : 0ULL;
+  const uint64_t fout = reg_tile<I>(states + it.state_off, base, geo, ops + it.ops_begin,
+                                    (int)it.nops, pool, lds, act, it.init != 0, nz);
+  // every tile of the instance computes the same frame; the first one stores it
+  if (it.noise && blockIdx.x == it.block_begin && threadIdx.x == 0) frames[it.frame_out] = fout;
+}
+
+}  // namespace tile
+}  // namespace qoa

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "qoa_internal.hpp"
 
@@ -39,9 +40,12 @@ struct FOp {
   uint8_t qphys[3] = {0, 0, 0};
 };
 
-inline void polar(double th, double* out) {  // Complex64::from_polar(1.0, th)
-  out[0] = 1.0 * std::cos(th);
-  out[1] = 1.0 * std::sin(th);
+// Complex64::from_polar(1.0, th), cos/sin from one sincos call (as api.cpp)
+inline void polar(double th, double* out) {
+  double c, s;
+  ::sincos(th, &s, &c);
+  out[0] = 1.0 * c;
+  out[1] = 1.0 * s;
 }
 
 inline uint64_t bit(int q) { return 1ULL << q; }
@@ -78,7 +82,8 @@ int canonical(uint32_t n, const qoa_gate& g, const double* mats, uint32_t flags,
       break;
     case QOA_OP_RX: case QOA_OP_RY: {
       if (!in(q0)) return 0;
-      const double c = std::cos(g.theta / 2.0), s = std::sin(g.theta / 2.0);
+      double c, s;
+      ::sincos(g.theta / 2.0, &s, &c);
       f.kind = FK_M; f.t = q0;
       if (g.op == QOA_OP_RX) {
         const double m[8] = {c, 0.0, 0.0, -s, 0.0, -s, c, 0.0};
@@ -526,6 +531,48 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
   }
   return QOA_OK;
 }
+
+}  // namespace qoa
+
+extern "C" int qoa_tile_op_size(void) { return (int)sizeof(qoa::TileOp); }
+
+extern "C" int qoa_plan_program(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
+                                const qoa_fuse_opts* opts, void* ops_out, uint32_t max_ops,
+                                uint8_t* spos_out, uint32_t* range_out, uint32_t max_passes,
+                                uint32_t* n_ops, uint32_t* n_passes, uint32_t* tile_k) {
+  using namespace qoa;
+  if (!g && ng) return set_error(QOA_ERR_ARG, "gates is NULL");
+  PlanOptions po{opts ? opts->tile_qubits : 0u, opts ? opts->low_qubits : 0u,
+                 opts ? opts->flags : 0u};
+  std::vector<double> zeros;
+  if (!mats) {
+    uint64_t maxaux = 0;
+    for (uint32_t i = 0; i < ng; ++i)
+      if (g[i].op == QOA_OP_U2 || g[i].op == QOA_OP_U4)
+        maxaux = std::max<uint64_t>(maxaux, g[i].aux + 1);
+    zeros.assign(32 * maxaux + 32, 0.0);
+    mats = zeros.data();
+  }
+  FusedPlan plan;
+  int rc = build_plan(n, g, ng, mats, po, &plan);
+  if (rc) return rc;
+  if (n_ops) *n_ops = (uint32_t)plan.ops.size();
+  if (n_passes) *n_passes = (uint32_t)plan.passes.size();
+  if (tile_k) *tile_k = plan.k;
+  if (ops_out && max_ops >= plan.ops.size())
+    std::memcpy(ops_out, plan.ops.data(), sizeof(TileOp) * plan.ops.size());
+  if (spos_out && range_out && max_passes >= plan.passes.size()) {
+    for (size_t p = 0; p < plan.passes.size(); ++p) {
+      std::memcpy(spos_out + 16 * p, plan.passes[p].spos, 16);
+      range_out[3 * p] = plan.passes[p].op_begin;
+      range_out[3 * p + 1] = plan.passes[p].op_end;
+      range_out[3 * p + 2] = plan.passes[p].reg_mode;
+    }
+  }
+  return QOA_OK;
+}
+
+namespace qoa {
 
 int plan_order(uint32_t n, const qoa_gate* g, uint32_t ng, const PlanOptions& opt,
                std::vector<uint32_t>* order, uint32_t* n_passes) {

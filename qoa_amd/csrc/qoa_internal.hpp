@@ -124,6 +124,11 @@ int launch_reset_apply(qoa_state* s, uint64_t bit, int slot);
 int launch_probabilities(qoa_state* s, double* dev_out);
 int launch_convert(qoa_state* dst, const qoa_state* src);
 int launch_upload_c64(qoa_state* s, const double* dev_src);
+// sample.hip: `count` samples from each of `ntraj` n-qubit states `stride`
+// amplitudes apart; results XOR the state's X frame (frames may be NULL).
+int launch_sample(hipStream_t st, const double2* states, uint64_t stride, uint32_t n,
+                  uint32_t ntraj, uint64_t seed, uint32_t count, const uint64_t* frames,
+                  uint32_t frame_stride, double* cs, uint64_t* out);
 int launch_download_c64(qoa_state* s, double* dev_dst);
 
 // fused passes (fused.hip / planner.cpp)

@@ -58,8 +58,10 @@ def test_mixed_gate_surface_fused(n, tile, flags):
 @pytest.mark.parametrize("n", [1, 2, 5, 8, 9, 12, 13])
 def test_whole_state_single_tile(n):
     g = W.layered_random(n, 12, seed=n, ladders=True)
+    order, passes = plan_fused(n, g)
+    assert passes == 1
     ref = po.state(n)
-    po.run_gates(ref, g)
+    po.run_gates(ref, g, order=order)
     with StateVector(n) as sv:
         sv.run_fused(g)
         assert np.array_equal(sv.download(), ref)

@@ -94,3 +94,50 @@ def test_plan_pass_counts_28q():
     g = W.layered_random(28, 40, ladders=True)
     _, passes = plan_fused(28, g, tile_qubits=12, low_qubits=3)
     assert passes < len(g) / 8, passes
+
+
+# ---- emulated op programs (CPU): the planner's output is checked as the
+# device kernels execute it, against the oracle run in the planned order ----
+@pytest.mark.parametrize("flags", [0, QOA_CZ_REF_MASKS, 2, 2 | QOA_CZ_REF_MASKS])
+@pytest.mark.parametrize("n,tile", [(9, 0), (12, 8), (11, 10)])
+def test_emulated_program_matches_oracle(n, tile, flags):
+    from tile_emulator import plan_program, run_program
+
+    g, mats = mixed_circuit(n, 250, seed=n * 3 + tile)
+    g = g[g["op"] != OP["U4"]]
+    a0 = random_state(n, seed=n + tile)
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=tile, flags=flags, matrices=mats)
+    order, _ = plan_fused(n, g, tile_qubits=tile, flags=flags)
+    ref = a0.copy()
+    po.run_gates(ref, g, flags=flags, matrices=mats, order=order)
+    a = a0.copy()
+    run_program(a, n, ops, spos, rng, k)
+    assert np.array_equal(a, ref)
+
+
+@pytest.mark.parametrize("tile", [8, 12])
+def test_emulated_ansatz_and_layers(tile):
+    from tile_emulator import plan_program, run_program
+
+    for n, g in ((12, W.ansatz_job(900, n_min=12, n_max=12, d_min=2, d_max=12)[2]),
+                 (13, W.layered_random(13, 10, seed=4, ladders=True))):
+        ops, spos, rng, k = plan_program(n, g, tile_qubits=tile)
+        order, _ = plan_fused(n, g, tile_qubits=tile)
+        ref = po.state(n)
+        po.run_gates(ref, g, order=order)
+        a = po.state(n)
+        run_program(a, n, ops, spos, rng, k)
+        assert np.array_equal(a, ref)
+
+
+def test_sincos_coefficients_match_oracle():
+    """Gate coefficients come from sincos on both sides (it can differ from
+    separate cos/sin in the last bit, e.g. at theta = 3.3790254426509847)."""
+    from tile_emulator import plan_program
+
+    th = 3.3790254426509847
+    g = np.array([(OP["RY"], 0, 0, 0, th, 0)], dtype=GATE_DTYPE)
+    ops, _, _, _ = plan_program(1, g, tile_qubits=4)
+    a = np.array([1, 0], dtype=complex)
+    po.orc.ry(a, 0, th)
+    assert ops[0]["c"][0] == a[0].real and ops[0]["c"][4] == a[1].real

@@ -1,0 +1,134 @@
+"""CPU emulator of the fused-pass op programs (test infrastructure).
+
+Runs a plan produced by qoa_plan_program exactly as the device kernels define
+it: per pass, per tile (base = tile index spread over the bits outside S),
+every op on the tile's local indices with the per-gate arithmetic (products
+written out so each multiply/add rounds on its own, as the kernels built
+without FMA do). Register-mode programs are checked for consistency: a 2x2,
+SWAP or MAT4 op's register slot must hold its target bit under the current
+layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from qoa_amd import _native as N
+from qoa_amd.state import gates_array
+
+TILE_OP = np.dtype([("kind", "u1"), ("t", "u1"), ("t2", "u1"), ("nins", "u1"), ("ins", "u1", 4),
+                    ("lmask", "<u4"), ("aux", "<u4"), ("gmask", "<u8"), ("gval", "<u8"),
+                    ("c", "<f8", 8), ("gi", "<u4"), ("nq", "u1"), ("psel", "u1"),
+                    ("qphys", "u1", 3), ("pad", "u1", 7)])
+TK_H, TK_M, TK_X, TK_Y, TK_DNEG, TK_DPHASE, TK_RZ, TK_SWAP, TK_MAT4, TK_LAYOUT = range(10)
+
+
+def plan_program(n, gates, tile_qubits=0, low_qubits=0, flags=0, matrices=None):
+    lib = N.load()
+    f = lib.qoa_plan_program
+    f.restype = C.c_int
+    assert lib.qoa_tile_op_size() == TILE_OP.itemsize
+    g = gates_array(gates)
+    m = None if matrices is None else np.ascontiguousarray(matrices, dtype=np.complex128)
+    opts = N.QoaFuseOpts(tile_qubits, low_qubits, flags, 0)
+    nops, npass, k = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    mp = None if m is None else N.ptr(m)
+    N.check(f(C.c_uint32(n), N.ptr(g), C.c_uint32(len(g)), mp, C.byref(opts), None,
+              C.c_uint32(0), None, None, C.c_uint32(0), C.byref(nops), C.byref(npass),
+              C.byref(k)))
+    ops = np.zeros(nops.value, dtype=TILE_OP)
+    spos = np.zeros((npass.value, 16), dtype=np.uint8)
+    rng = np.zeros((npass.value, 3), dtype=np.uint32)
+    N.check(f(C.c_uint32(n), N.ptr(g), C.c_uint32(len(g)), mp, C.byref(opts), N.ptr(ops),
+              C.c_uint32(len(ops)), N.ptr(spos), N.ptr(rng), C.c_uint32(npass.value),
+              C.byref(nops), C.byref(npass), C.byref(k)))
+    return ops, spos, rng, k.value
+
+
+def _cmul(xr, xi, yr, yi):
+    return xr * yr - xi * yi, xr * yi + xi * yr
+
+
+def _deposit(x: int, positions) -> int:
+    for b in positions:
+        lo = x & ((1 << b) - 1)
+        x = ((x >> b) << (b + 1)) | lo
+    return x
+
+
+def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int) -> None:
+    """Applies the program to the complex128 state `a` in place."""
+    re, im = a.real.copy(), a.imag.copy()
+    T = 1 << k
+    loc = np.arange(T)
+    for p in range(len(rng)):
+        pos = [int(x) for x in spos[p][:k]]
+        off = np.zeros(T, dtype=np.int64)
+        for j, b in enumerate(pos):
+            off |= ((loc >> j) & 1) << b
+        b0, b1, reg_mode = (int(x) for x in rng[p])
+        layout = list(range(k - 4, k))
+        for tile in range(1 << (n - k)):
+            base = _deposit(tile, pos)
+            idx = base | off
+            vr, vi = re[idx].copy(), im[idx].copy()
+            for op in ops[b0:b1]:
+                kind = int(op["kind"])
+                if kind == TK_LAYOUT:
+                    layout = [int(x) for x in op["ins"]]
+                    continue
+                if (base & int(op["gmask"])) != int(op["gval"]):
+                    continue
+                lm = int(op["lmask"])
+                c = op["c"]
+                if kind in (TK_H, TK_M, TK_X, TK_Y):
+                    t = int(op["t"])
+                    if reg_mode:
+                        assert layout[int(op["ins"][0])] == t, "register slot mismatch"
+                    l0 = loc[((loc >> t) & 1 == 0) & ((loc & lm) == lm)]
+                    l1 = l0 | (1 << t)
+                    xr, xi, yr, yi = vr[l0], vi[l0], vr[l1], vi[l1]
+                    if kind == TK_H:
+                        o0 = _cmul(c[0], c[1], xr + yr, xi + yi)
+                        o1 = _cmul(c[0], c[1], xr - yr, xi - yi)
+                    elif kind == TK_M:
+                        a0 = _cmul(c[0], c[1], xr, xi)
+                        a1 = _cmul(c[2], c[3], yr, yi)
+                        b0_ = _cmul(c[4], c[5], xr, xi)
+                        b1_ = _cmul(c[6], c[7], yr, yi)
+                        o0 = (a0[0] + a1[0], a0[1] + a1[1])
+                        o1 = (b0_[0] + b1_[0], b0_[1] + b1_[1])
+                    elif kind == TK_X:
+                        o0, o1 = (yr, yi), (xr, xi)
+                    else:
+                        o0 = _cmul(yr, yi, 0.0, -1.0)
+                        o1 = _cmul(xr, xi, 0.0, 1.0)
+                    vr[l0], vi[l0] = o0
+                    vr[l1], vi[l1] = o1
+                elif kind in (TK_DNEG, TK_DPHASE):
+                    sel = loc[(loc & lm) == lm]
+                    if kind == TK_DNEG:
+                        vr[sel], vi[sel] = vr[sel] * -1.0, vi[sel] * -1.0
+                    else:
+                        vr[sel], vi[sel] = _cmul(vr[sel], vi[sel], c[0], c[1])
+                elif kind == TK_RZ:
+                    t = int(op["t"])
+                    one = (loc >> t) & 1 == 1
+                    r1, i1 = _cmul(vr, vi, c[2], c[3])
+                    r0, i0 = _cmul(vr, vi, c[0], c[1])
+                    vr, vi = np.where(one, r1, r0), np.where(one, i1, i0)
+                elif kind == TK_SWAP:
+                    t, t2 = int(op["t"]), int(op["t2"])
+                    if reg_mode:
+                        assert {layout[int(op["ins"][0])], layout[int(op["ins"][1])]} == {t, t2}
+                    m1, m2 = 1 << t, 1 << t2
+                    l = loc[((loc & (m1 | m2)) == 0) & ((loc & lm) == lm)]
+                    ur, ui = vr[l | m2].copy(), vi[l | m2].copy()
+                    vr[l | m2], vi[l | m2] = vr[l | m1], vi[l | m1]
+                    vr[l | m1], vi[l | m1] = ur, ui
+                else:
+                    raise NotImplementedError("MAT4 is covered by the GPU tests")
+            re[idx], im[idx] = vr, vi
+    a.real[:] = re
+    a.imag[:] = im
