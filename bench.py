@@ -34,6 +34,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA = 2, 4
 
 
 def log(msg: str) -> None:
@@ -74,7 +75,9 @@ def run_circuit(sv, gates, method: str, steps: int, warmup: int, tag: str):
         if method == "per-gate":
             sv.run_gates(gates)
         elif method == "fused-lds":
-            sv.run_fused(gates, flags=2)
+            sv.run_fused(gates, flags=QOA_FUSE_LDS_ONLY)
+        elif method == "fused-fma":
+            sv.run_fused(gates, flags=QOA_FUSE_FMA)
         else:
             sv.run_fused(gates)
 
@@ -170,7 +173,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--method", default="fused", choices=["fused", "fused-lds", "per-gate"])
+    ap.add_argument("--method", default="fused",
+                    choices=["fused", "fused-fma", "fused-lds", "per-gate"])
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=20241203)
@@ -208,7 +212,7 @@ def main() -> int:
 
     extras = {}
     if not args.no_extras:
-        for other in ("per-gate", "fused-lds"):
+        for other in ("fused", "fused-fma", "per-gate", "fused-lds"):
             if other == args.method:
                 continue
             el, kk = run_circuit(sv, gates, other, 1, 0, f"c2 {other}")

@@ -88,6 +88,11 @@ typedef struct qoa_gate {
 #define QOA_FUSE_LDS_ONLY 2u /* fused passes keep the whole tile in LDS and
                                 apply every gate there (no register-resident
                                 segments); for comparison and testing.      */
+#define QOA_FUSE_FMA 4u      /* register-resident passes compute complex
+                                products with fused multiply-adds: fewer
+                                FP64 instructions, results within a few ulp
+                                of the exact (num-complex) arithmetic
+                                instead of bit-identical to it.              */
 #define QOA_CZ_REF_MASKS 1u  /* CZ/CPHASE use the integer c|t as the bit mask,
                                 as QuantumState::apply_cz does
                                 (src/runtime/quantum_state.rs:338, 356-361).

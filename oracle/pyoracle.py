@@ -46,11 +46,13 @@ _SIGS = {
     "orc_traj_pauli": [U64, U32, U32, U32, D],
     "orc_run_trajectory": [P, U64, P, U32, P, U32, U64, U32, D, D],
     "orc_run_trajectory_ordered": [P, U64, P, P, U32, P, U32, U64, U32, D, D],
+    "orc_sample_uniform": [U64, U32, U32], "orc_sample": [P, U64, D],
 }
 _RES = {"orc_normalize": I32, "orc_measure": I32, "orc_run_gates": I32,
         "orc_run_gates_ordered": I32, "orc_splitmix64_next": U64, "orc_rng_u64": U64,
         "orc_rng_f64": D, "orc_norm_sqr": D, "orc_qs_apply": I32, "orc_qs_measure": I32,
-        "orc_traj_pauli": U32, "orc_run_trajectory": I32, "orc_run_trajectory_ordered": I32}
+        "orc_traj_pauli": U32, "orc_run_trajectory": I32, "orc_run_trajectory_ordered": I32,
+        "orc_sample_uniform": D, "orc_sample": U64}
 
 _lib = None
 

@@ -27,6 +27,8 @@ QOA_ERR_UNSUPPORTED = -7
 QOA_C128 = 0
 QOA_C64 = 1
 QOA_CZ_REF_MASKS = 1
+QOA_FUSE_LDS_ONLY = 2
+QOA_FUSE_FMA = 4
 
 NOISE_NONE, NOISE_IDEAL, NOISE_RANDOM, NOISE_FIXED = 0, 1, 2, 3
 

@@ -28,6 +28,10 @@ COMMON = [
     "-ffp-contract=off",
     # Packed FP32 instructions stay off for every kernel in this library.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # The register-resident tile keeps its 16 amplitudes in VGPRs only if
+    # switch arms that differ in register indices are not merged (merging
+    # them leaves a dynamically indexed array in scratch memory).
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-Wall", "-Wno-unused-result", "-Wno-unused-command-line-argument",
     "-I", str(PKG.parent / "include"),
 ]

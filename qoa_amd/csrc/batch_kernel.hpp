@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_reg(double2* __restrict
   nz.p2 = it.p2;
   nz.traj = it.traj;
   nz.frame = (it.noise && !it.init) ? frames[it.frame_in] : 0ULL;
-  const uint64_t fout = reg_tile<I>(states + it.state_off, base, geo, ops + it.ops_begin,
+  const uint64_t fout = reg_tile<I, false>(states + it.state_off, base, geo, ops + it.ops_begin,
                                     (int)it.nops, pool, lds, act, it.init != 0, nz);
   // every tile of the instance computes the same frame; the first one stores it
   if (it.noise && blockIdx.x == it.block_begin && threadIdx.x == 0) frames[it.frame_out] = fout;

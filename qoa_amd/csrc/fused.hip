@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kThreads) void k_fused(double2* __restrict__ a,
   for (uint32_t l = tid; l < T; l += kThreads) a[base | scatter<I>(l, geo)] = lds[l];
 }
 
-template <typename I>
+template <typename I, bool F>
 __global__ __launch_bounds__(512) void k_fused_reg(double2* __restrict__ a,
                                                   const TileOp* __restrict__ ops, int nops,
                                                   const double* __restrict__ pool, Geo geo) {
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(512) void k_fused_reg(double2* __restrict__ a,
     const int b = geo.spos[j];
     base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
   }
-  reg_tile<I>(a, base, geo, ops, nops, pool, lds, true, false, NoiseArgs{});
+  reg_tile<I, F>(a, base, geo, ops, nops, pool, lds, true, false, NoiseArgs{});
 }
 
 // FNV-1a over the bytes that determine a plan.
@@ -235,9 +235,13 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
     (void)hipFuncSetAttribute((const void*)k_fused<uint64_t>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
-    (void)hipFuncSetAttribute((const void*)k_fused_reg<uint32_t>,
+    (void)hipFuncSetAttribute((const void*)k_fused_reg<uint32_t, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
-    (void)hipFuncSetAttribute((const void*)k_fused_reg<uint64_t>,
+    (void)hipFuncSetAttribute((const void*)k_fused_reg<uint64_t, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
+    (void)hipFuncSetAttribute((const void*)k_fused_reg<uint32_t, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
+    (void)hipFuncSetAttribute((const void*)k_fused_reg<uint64_t, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
   });
   const uint64_t tiles = 1ULL << (s->n - k);
@@ -253,12 +257,22 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
       // 16 amplitudes per thread; LDS holds the padded tile for layout changes
       const unsigned threads = 1u << (k - kRegBits);
       const size_t lbytes = sizeof(double2) * ((1u << k) + (1u << k) / 16);
-      if (use32(s))
-        hipLaunchKernelGGL((k_fused_reg<uint32_t>), dim3((unsigned)tiles), dim3(threads), lbytes,
-                           s->stream, a, ops, nops, c->d_pool, geo);
-      else
-        hipLaunchKernelGGL((k_fused_reg<uint64_t>), dim3((unsigned)tiles), dim3(threads), lbytes,
-                           s->stream, a, ops, nops, c->d_pool, geo);
+      const bool fma = (opt.flags & QOA_FUSE_FMA) != 0;
+      const dim3 grid((unsigned)tiles), block(threads);
+      if (use32(s)) {
+        if (fma)
+          hipLaunchKernelGGL((k_fused_reg<uint32_t, true>), grid, block, lbytes, s->stream, a, ops,
+                             nops, c->d_pool, geo);
+        else
+          hipLaunchKernelGGL((k_fused_reg<uint32_t, false>), grid, block, lbytes, s->stream, a,
+                             ops, nops, c->d_pool, geo);
+      } else if (fma) {
+        hipLaunchKernelGGL((k_fused_reg<uint64_t, true>), grid, block, lbytes, s->stream, a, ops,
+                           nops, c->d_pool, geo);
+      } else {
+        hipLaunchKernelGGL((k_fused_reg<uint64_t, false>), grid, block, lbytes, s->stream, a, ops,
+                           nops, c->d_pool, geo);
+      }
     } else if (use32(s)) {
       hipLaunchKernelGGL((k_fused<uint32_t>), dim3((unsigned)tiles), dim3(kThreads), lds_bytes,
                          s->stream, a, ops, nops, c->d_pool, geo);

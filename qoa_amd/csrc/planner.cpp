@@ -229,7 +229,7 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
     }
     // with noise, an X error on an outside qubit changes the tile's frame, so
     // ops that read or hit outside qubits keep their relative order
-    op.pad[0] = outside ? 1 : 0;
+    op.outside = outside ? 1 : 0;
   }
   const uint64_t pin = f.pmask & S, pout = f.pmask & ~S;
   op.lmask = local_bits(pin, spos, k);
@@ -312,7 +312,7 @@ inline SegOp seg_masks_local(const TileOp& op) {
 
 inline SegOp seg_masks(const TileOp& op) {
   SegOp m = seg_masks_local(op);
-  if (op.pad[0]) m.all |= 1u << 31;  // ordered against other outside-qubit ops
+  if (op.outside) m.all |= 1u << 31;  // ordered against other outside-qubit ops
   return m;
 }
 
@@ -377,6 +377,16 @@ inline uint8_t slot_of(uint32_t R, uint32_t bitpos) {
   return (uint8_t)__builtin_popcount(R & ((1u << bitpos) - 1u));
 }
 
+// Splits an op's controls into register slots and thread bits for layout R.
+void reg_fields(TileOp* op, uint32_t R) {
+  op->rsel = 0;
+  for (uint32_t b = 0; b < 32; ++b)
+    if ((op->lmask & R) & (1u << b)) op->rsel |= (uint8_t)(1u << slot_of(R, b));
+  op->tl = op->lmask & ~R;
+  op->rt = 0xFF;
+  if (op->kind == TK_RZ && (R & (1u << op->t))) op->rt = slot_of(R, op->t);
+}
+
 void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& gis, uint32_t k,
                   std::vector<TileOp>* out, std::vector<uint32_t>* order) {
   const uint32_t L0 = ((1u << kRegBits) - 1u) << (k - kRegBits);
@@ -405,6 +415,7 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
           op.ins[0] = slot_of(R, op.t);
           if (op.kind == TK_SWAP || op.kind == TK_MAT4) op.ins[1] = slot_of(R, op.t2);
         }
+        reg_fields(&op, R);
         out->push_back(op);
         if (gis[i] != last_gi) {
           order->push_back(gis[i]);

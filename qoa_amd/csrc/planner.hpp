@@ -50,7 +50,13 @@ struct alignas(16) TileOp {
   uint32_t gi;
   uint8_t nq, psel;
   uint8_t qphys[3];
-  uint8_t pad[5];
+  // Register mode only (filled per layout by the segment scheduler): the
+  // control bits of lmask split into register slots (rsel: slot j set means
+  // local bit ins-of-layout[j] must be 1) and thread bits (tl = lmask & ~R);
+  // rt is the register slot of a TK_RZ target, 0xFF when it is a thread bit.
+  uint8_t rsel, rt;
+  uint8_t outside;  // noise: the op reads or hits a qubit outside the tile
+  uint32_t tl;
 };
 static_assert(sizeof(TileOp) == 112, "TileOp layout");
 

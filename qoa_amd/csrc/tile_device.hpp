@@ -2,7 +2,9 @@
 //
 // Arithmetic follows the per-gate sweeps exactly (num-complex products, no
 // FMA: every includer is built with -ffp-contract=off), so a gate applied in
-// a tile and by its own sweep give the same bits.
+// a tile and by its own sweep give the same bits. The register-resident pass
+// also has an FMA instantiation (F = true) whose complex products use fused
+// multiply-adds; it is selected per call and is not bit-identical.
 #pragma once
 #include "planner.hpp"
 #include "qoa_internal.hpp"
@@ -45,33 +47,104 @@ __device__ __forceinline__ I scatter(uint32_t l, const Geo& g) {
   return off;
 }
 
+// ---- complex products, exact (F = false) or with FMA (F = true) -------------
+template <bool F>
+__device__ __forceinline__ double2 mulc(double2 x, double2 y) {
+  if (F) return make_double2(__builtin_fma(x.x, y.x, -(x.y * y.y)), __builtin_fma(x.x, y.y, x.y * y.x));
+  return cmul(x, y);
+}
+
+// a*x + b*y
+template <bool F>
+__device__ __forceinline__ double2 mul2c(double2 a, double2 x, double2 b, double2 y) {
+  if (F) {
+    double re = __builtin_fma(a.x, x.x, -(a.y * x.y));
+    re = __builtin_fma(b.x, y.x, re);
+    re = __builtin_fma(-b.y, y.y, re);
+    double im = __builtin_fma(a.x, x.y, a.y * x.x);
+    im = __builtin_fma(b.x, y.y, im);
+    im = __builtin_fma(b.y, y.x, im);
+    return make_double2(re, im);
+  }
+  return cadd(cmul(a, x), cmul(b, y));
+}
+
+// acc + a*x
+template <bool F>
+__device__ __forceinline__ double2 macc(double2 acc, double2 a, double2 x) {
+  if (F) {
+    return make_double2(__builtin_fma(-a.y, x.y, __builtin_fma(a.x, x.x, acc.x)),
+                        __builtin_fma(a.y, x.x, __builtin_fma(a.x, x.y, acc.y)));
+  }
+  return cadd(acc, cmul(a, x));
+}
 
 // ---- register-resident passes -------------------------------------------------
 // Thread t of a 2^(k-4)-thread block holds the 16 amplitudes whose local
-// indices are thr_l(t) | regl[r], r = 0..15: four "register bits" of the
-// local index come from r, the others from t. Ops whose mixed bits are
-// register bits (and every diagonal op) run on registers; a TK_LAYOUT op
-// changes the register bits through one LDS round trip.
+// indices are thr_l(t) | regl(r), r = 0..15: slot j of the register index r
+// stands for local bit rpos[j] and the thread index fills the other local
+// bits in ascending order. Ops whose mixed bits are register bits (and every
+// diagonal op) run on registers; a TK_LAYOUT op changes the register bits
+// through one LDS round trip. Controls come split by the planner: register
+// slots (rsel, a test that is uniform over the block per register) and
+// thread bits (tl, one per-thread test per op).
 constexpr int kR = kRegAmps;
 
 __device__ __forceinline__ uint32_t lpad(uint32_t l) { return l + (l >> 4); }
 
-template <int J, int KIND>
-__device__ __forceinline__ void reg_pair(double2 (&v)[kR], const TileOp& op, uint32_t thr_l,
-                                         const uint32_t (&regl)[kR]) {
+struct RegLayout {
+  uint32_t rb[kRegBits];  // 1 << rpos[j] (uniform)
+  uint32_t thr_l;         // this thread's local bits, register bits clear
+};
+
+__device__ __forceinline__ uint32_t regl(const RegLayout& L, int r) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j)
+    if ((r >> j) & 1) o |= L.rb[j];
+  return o;
+}
+
+__device__ __forceinline__ void set_layout(RegLayout& L, uint32_t tid, int k, const uint8_t* rpos) {
+  uint32_t rmask = 0;
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) {
+    L.rb[j] = 1u << rpos[j];
+    rmask |= L.rb[j];
+  }
+  uint32_t l = 0, src = 0;
+  for (int b = 0; b < k; ++b) {
+    if (rmask & (1u << b)) continue;
+    l |= ((tid >> src) & 1u) << b;
+    ++src;
+  }
+  L.thr_l = l;
+}
+
+constexpr int kHReal = 16;  // TK_H with a real factor: a' = (nf*Re, nf*Im)
+
+template <int J, int KIND, bool F>
+__device__ __forceinline__ void reg_pair(double2 (&v)[kR], const TileOp& op) {
+  const uint32_t rsel = op.rsel;
+  const double2 c0 = make_double2(op.c[0], op.c[1]), c1 = make_double2(op.c[2], op.c[3]);
+  const double2 c2 = make_double2(op.c[4], op.c[5]), c3 = make_double2(op.c[6], op.c[7]);
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & (1 << J)) continue;
+    if (((uint32_t)r & rsel) != rsel) continue;  // uniform
     const int r1 = r | (1 << J);
-    if (((thr_l | regl[r]) & op.lmask) != op.lmask) continue;
     const double2 x = v[r], y = v[r1];
-    if (KIND == TK_H) {
-      const double2 nf = make_double2(op.c[0], op.c[1]);
-      v[r] = cmul(nf, cadd(x, y));
-      v[r1] = cmul(nf, csub(x, y));
+    if (KIND == kHReal) {
+      // equals cmul((nf, 0), .) up to the sign of zero terms
+      const double nf = c0.x;
+      v[r] = make_double2(nf * (x.x + y.x), nf * (x.y + y.y));
+      v[r1] = make_double2(nf * (x.x - y.x), nf * (x.y - y.y));
+    } else if (KIND == TK_H) {
+      v[r] = mulc<F>(c0, cadd(x, y));
+      v[r1] = mulc<F>(c0, csub(x, y));
     } else if (KIND == TK_M) {
-      v[r] = cadd(cmul(make_double2(op.c[0], op.c[1]), x), cmul(make_double2(op.c[2], op.c[3]), y));
-      v[r1] = cadd(cmul(make_double2(op.c[4], op.c[5]), x), cmul(make_double2(op.c[6], op.c[7]), y));
+      v[r] = mul2c<F>(c0, x, c1, y);
+      v[r1] = mul2c<F>(c2, x, c3, y);
     } else if (KIND == TK_X) {
       v[r] = y;
       v[r1] = x;
@@ -82,24 +155,22 @@ __device__ __forceinline__ void reg_pair(double2 (&v)[kR], const TileOp& op, uin
   }
 }
 
-template <int KIND>
-__device__ __forceinline__ void reg_pair_j(double2 (&v)[kR], const TileOp& op, uint32_t thr_l,
-                                           const uint32_t (&regl)[kR]) {
+template <int KIND, bool F>
+__device__ __forceinline__ void reg_pair_j(double2 (&v)[kR], const TileOp& op) {
   switch (op.ins[0]) {
-    case 0: reg_pair<0, KIND>(v, op, thr_l, regl); break;
-    case 1: reg_pair<1, KIND>(v, op, thr_l, regl); break;
-    case 2: reg_pair<2, KIND>(v, op, thr_l, regl); break;
-    default: reg_pair<3, KIND>(v, op, thr_l, regl); break;
+    case 0: reg_pair<0, KIND, F>(v, op); break;
+    case 1: reg_pair<1, KIND, F>(v, op); break;
+    case 2: reg_pair<2, KIND, F>(v, op); break;
+    default: reg_pair<3, KIND, F>(v, op); break;
   }
 }
 
 template <int A, int B>
-__device__ __forceinline__ void reg_swap(double2 (&v)[kR], const TileOp& op, uint32_t thr_l,
-                                         const uint32_t (&regl)[kR]) {
+__device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel) {
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & ((1 << A) | (1 << B))) continue;
-    if (((thr_l | regl[r]) & op.lmask) != op.lmask) continue;
+    if (((uint32_t)r & rsel) != rsel) continue;
     const double2 u = v[r | (1 << B)];
     v[r | (1 << B)] = v[r | (1 << A)];
     v[r | (1 << A)] = u;
@@ -107,47 +178,41 @@ __device__ __forceinline__ void reg_swap(double2 (&v)[kR], const TileOp& op, uin
 }
 
 // A < B are the register slots of the lower and higher local bit.
-template <int A, int B>
+template <int A, int B, bool F>
 __device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u) {
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & ((1 << A) | (1 << B))) continue;
-    const int idx[4] = {r, r | (1 << A), r | (1 << B), r | (1 << A) | (1 << B)};
-    double2 w[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) w[q] = v[idx[q]];
+    const double2 w0 = v[r], w1 = v[r | (1 << A)], w2 = v[r | (1 << B)];
+    const double2 w3 = v[r | (1 << A) | (1 << B)];
+    double2 o[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      double2 acc = cmul(make_double2(u[8 * s], u[8 * s + 1]), w[0]);
-#pragma unroll
-      for (int q = 1; q < 4; ++q)
-        acc = cadd(acc, cmul(make_double2(u[8 * s + 2 * q], u[8 * s + 2 * q + 1]), w[q]));
-      v[idx[s]] = acc;
+      double2 acc = mulc<F>(make_double2(u[8 * s], u[8 * s + 1]), w0);
+      acc = macc<F>(acc, make_double2(u[8 * s + 2], u[8 * s + 3]), w1);
+      acc = macc<F>(acc, make_double2(u[8 * s + 4], u[8 * s + 5]), w2);
+      acc = macc<F>(acc, make_double2(u[8 * s + 6], u[8 * s + 7]), w3);
+      o[s] = acc;
     }
+    v[r] = o[0];
+    v[r | (1 << A)] = o[1];
+    v[r | (1 << B)] = o[2];
+    v[r | (1 << A) | (1 << B)] = o[3];
   }
 }
 
-// Local index bits of thread t for register bits R (t's bits fill the other
-// local positions in ascending order) and the register offsets regl[r].
-__device__ __forceinline__ void make_layout(uint32_t tid, int k, const uint8_t* rpos,
-                                            uint32_t& thr_l, uint32_t (&regl)[kR]) {
-  uint32_t rmask = 0;
+// Writes the registers to LDS under layout L (padded rows: no bank conflicts
+// for the strided layouts).
+__device__ __forceinline__ void regs_to_lds(const double2 (&v)[kR], const RegLayout& L,
+                                            double2* lds, uint32_t xl = 0) {
 #pragma unroll
-  for (int j = 0; j < kRegBits; ++j) rmask |= 1u << rpos[j];
-  uint32_t l = 0, src = 0;
-  for (int b = 0; b < k; ++b) {
-    if (rmask & (1u << b)) continue;
-    l |= ((tid >> src) & 1u) << b;
-    ++src;
-  }
-  thr_l = l;
+  for (int r = 0; r < kR; ++r) lds[lpad((L.thr_l | regl(L, r)) ^ xl)] = v[r];
+}
+
+__device__ __forceinline__ void lds_to_regs(double2 (&v)[kR], const RegLayout& L,
+                                            const double2* lds) {
 #pragma unroll
-  for (int r = 0; r < kR; ++r) {
-    uint32_t o = 0;
-#pragma unroll
-    for (int j = 0; j < kRegBits; ++j) o |= (uint32_t)((r >> j) & 1) << rpos[j];
-    regl[r] = o;
-  }
+  for (int r = 0; r < kR; ++r) v[r] = lds[lpad(L.thr_l | regl(L, r))];
 }
 
 // Trajectory noise state of one tile pass (on == 0: no noise).
@@ -205,18 +270,17 @@ __device__ __forceinline__ void reg_flip(double2 (&v)[kR]) {
 // Pauli `code` (1 X, 2 Y, 3 Z) on local bit b of a register-resident tile,
 // with the arithmetic of apply_{x,y,z}_simd (vectorization.rs:942-999).
 __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint32_t b,
-                                          const uint8_t* rpos, uint32_t thr_l,
-                                          const uint32_t (&regl)[kR], double2* lds, bool act) {
+                                          const RegLayout& L, double2* lds, bool act) {
   if (code == 3) {
 #pragma unroll
     for (int r = 0; r < kR; ++r)
-      if (((thr_l | regl[r]) >> b) & 1u) v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
+      if (((L.thr_l | regl(L, r)) >> b) & 1u) v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
     return;
   }
   int slot = -1;
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j)
-    if (rpos[j] == b) slot = j;
+    if (L.rb[j] == (1u << b)) slot = j;
   if (slot >= 0) {
     const bool y = code == 2;
     switch (slot) {
@@ -228,21 +292,84 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
     return;
   }
   // b is a thread bit: exchange partners through LDS
-  if (act) {
-#pragma unroll
-    for (int r = 0; r < kR; ++r) lds[lpad(thr_l | regl[r])] = v[r];
-  }
+  if (act) regs_to_lds(v, L, lds, 1u << b);
   __syncthreads();
-  if (act) {
-#pragma unroll
-    for (int r = 0; r < kR; ++r) v[r] = lds[lpad((thr_l | regl[r]) ^ (1u << b))];
-  }
+  if (act) lds_to_regs(v, L, lds);
   __syncthreads();
   if (code == 2) {
+    const double2 f = ((L.thr_l >> b) & 1u) ? make_double2(0.0, 1.0) : make_double2(0.0, -1.0);
 #pragma unroll
-    for (int r = 0; r < kR; ++r)
-      v[r] = (((thr_l | regl[r]) >> b) & 1u) ? cmul(v[r], make_double2(0.0, 1.0))
-                                             : cmul(v[r], make_double2(0.0, -1.0));
+    for (int r = 0; r < kR; ++r) v[r] = cmul(v[r], f);
+  }
+}
+
+// Applies one non-layout op whose tile condition holds to this thread's
+// registers (the thread-bit controls have been checked by the caller).
+template <bool F>
+__device__ __forceinline__ void reg_op(double2 (&v)[kR], const TileOp& op, const RegLayout& L,
+                                       const double* __restrict__ pool) {
+  switch (op.kind) {
+    case TK_H:
+      if (op.c[1] == 0.0) reg_pair_j<kHReal, F>(v, op);
+      else reg_pair_j<TK_H, F>(v, op);
+      break;
+    case TK_M: reg_pair_j<TK_M, F>(v, op); break;
+    case TK_X: reg_pair_j<TK_X, F>(v, op); break;
+    case TK_Y: reg_pair_j<TK_Y, F>(v, op); break;
+    case TK_DNEG: {
+      const uint32_t rsel = op.rsel;
+#pragma unroll
+      for (int r = 0; r < kR; ++r)
+        if (((uint32_t)r & rsel) == rsel) v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
+    } break;
+    case TK_DPHASE: {
+      const uint32_t rsel = op.rsel;
+      const double2 f = make_double2(op.c[0], op.c[1]);
+#pragma unroll
+      for (int r = 0; r < kR; ++r)
+        if (((uint32_t)r & rsel) == rsel) v[r] = mulc<F>(v[r], f);
+    } break;
+    case TK_RZ: {
+      const double2 m00 = make_double2(op.c[0], op.c[1]);
+      const double2 m11 = make_double2(op.c[2], op.c[3]);
+      if (op.rt != 0xFF) {  // target held in register slot rt: the factor is uniform per r
+        const uint32_t j = op.rt;
+#pragma unroll
+        for (int r = 0; r < kR; ++r) v[r] = mulc<F>(v[r], (((uint32_t)r >> j) & 1u) ? m11 : m00);
+      } else {
+        const double2 f = ((L.thr_l >> op.t) & 1u) ? m11 : m00;
+#pragma unroll
+        for (int r = 0; r < kR; ++r) v[r] = mulc<F>(v[r], f);
+      }
+    } break;
+    case TK_SWAP: {
+      const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
+      const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
+      const uint32_t rsel = op.rsel;
+      switch (a0 * 4 + b0) {
+        case 1: reg_swap<0, 1>(v, rsel); break;
+        case 2: reg_swap<0, 2>(v, rsel); break;
+        case 3: reg_swap<0, 3>(v, rsel); break;
+        case 6: reg_swap<1, 2>(v, rsel); break;
+        case 7: reg_swap<1, 3>(v, rsel); break;
+        default: reg_swap<2, 3>(v, rsel); break;
+      }
+    } break;
+    case TK_MAT4: {
+      const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
+      const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
+      const double* u = pool + op.aux;
+      switch (a0 * 4 + b0) {
+        case 1: reg_mat4<0, 1, F>(v, u); break;
+        case 2: reg_mat4<0, 2, F>(v, u); break;
+        case 3: reg_mat4<0, 3, F>(v, u); break;
+        case 6: reg_mat4<1, 2, F>(v, u); break;
+        case 7: reg_mat4<1, 3, F>(v, u); break;
+        default: reg_mat4<2, 3, F>(v, u); break;
+      }
+    } break;
+    default:
+      break;
   }
 }
 
@@ -253,7 +380,7 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
 // (its phase, a tile constant, is applied at once), and the frame bits that
 // fall inside the tile are applied when the tile is loaded. Returns the frame
 // leaving the pass.
-template <typename I>
+template <typename I, bool F>
 __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, const Geo& geo,
                                              const TileOp* __restrict__ ops, int nops,
                                              const double* __restrict__ pool, double2* lds,
@@ -263,18 +390,28 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
   uint8_t rpos[kRegBits];
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(k - kRegBits + j);
-  uint32_t thr_l;
-  uint32_t regl[kR];
-  make_layout(tid, k, rpos, thr_l, regl);
-  const I gthr = scatter<I>(thr_l, geo);
+  RegLayout L;
+  set_layout(L, tid, k, rpos);  // thr_l = tid: thread bits are local bits 0..k-5
+  // HBM addresses: base | thread part | register part (uniform per r)
+  I gthr = 0;
+  for (int j = 0; j < k - kRegBits; ++j) gthr |= (I)((tid >> j) & 1u) << geo.spos[j];
+  I gb[kRegBits];
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) gb[j] = (I)1 << geo.spos[k - kRegBits + j];
+  auto gaddr = [&](int r) {
+    I o = base | gthr;
+#pragma unroll
+    for (int j = 0; j < kRegBits; ++j)
+      if ((r >> j) & 1) o |= gb[j];
+    return o;
+  };
   double2 v[kR];
   if (init_zero) {  // the state starts as |0...0>: nothing to read
 #pragma unroll
-    for (int r = 0; r < kR; ++r)
-      v[r] = make_double2((base | gthr | scatter<I>(regl[r], geo)) == 0 ? 1.0 : 0.0, 0.0);
+    for (int r = 0; r < kR; ++r) v[r] = make_double2(gaddr(r) == 0 ? 1.0 : 0.0, 0.0);
   } else if (act) {
 #pragma unroll
-    for (int r = 0; r < kR; ++r) v[r] = a[base | gthr | scatter<I>(regl[r], geo)];
+    for (int r = 0; r < kR; ++r) v[r] = a[gaddr(r)];
   }
   uint64_t frame = nz.on ? nz.frame : 0ULL;
   if (frame) {
@@ -285,15 +422,9 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
       if ((frame >> geo.spos[j]) & 1ULL) xl |= 1u << j;
     }
     if (xl) {  // frame bits inside the tile: apply those X flips now
-      if (act) {
-#pragma unroll
-        for (int r = 0; r < kR; ++r) lds[lpad(thr_l | regl[r])] = v[r];
-      }
+      if (act) regs_to_lds(v, L, lds, xl);
       __syncthreads();
-      if (act) {
-#pragma unroll
-        for (int r = 0; r < kR; ++r) v[r] = lds[lpad((thr_l | regl[r]) ^ xl)];
-      }
+      if (act) lds_to_regs(v, L, lds);
       __syncthreads();
       frame &= ~smask;
     }
@@ -301,74 +432,16 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
 
   for (int o = 0; o < nops; ++o) {
     const TileOp& op = ops[o];
-    if ((((uint64_t)base ^ frame) & op.gmask) == op.gval) {  // uniform over the tile
-      switch (op.kind) {
-        case TK_LAYOUT: {
-          if (act) {
+    if (op.kind == TK_LAYOUT) {
+      if (act) regs_to_lds(v, L, lds);
+      __syncthreads();
 #pragma unroll
-            for (int r = 0; r < kR; ++r) lds[lpad(thr_l | regl[r])] = v[r];
-          }
-          __syncthreads();
-#pragma unroll
-          for (int j = 0; j < kRegBits; ++j) rpos[j] = op.ins[j];
-          make_layout(tid, k, rpos, thr_l, regl);
-          if (act) {
-#pragma unroll
-            for (int r = 0; r < kR; ++r) v[r] = lds[lpad(thr_l | regl[r])];
-          }
-          __syncthreads();
-        } break;
-        case TK_H: reg_pair_j<TK_H>(v, op, thr_l, regl); break;
-        case TK_M: reg_pair_j<TK_M>(v, op, thr_l, regl); break;
-        case TK_X: reg_pair_j<TK_X>(v, op, thr_l, regl); break;
-        case TK_Y: reg_pair_j<TK_Y>(v, op, thr_l, regl); break;
-        case TK_DNEG: {
-#pragma unroll
-          for (int r = 0; r < kR; ++r)
-            if (((thr_l | regl[r]) & op.lmask) == op.lmask)
-              v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
-        } break;
-        case TK_DPHASE: {
-          const double2 f = make_double2(op.c[0], op.c[1]);
-#pragma unroll
-          for (int r = 0; r < kR; ++r)
-            if (((thr_l | regl[r]) & op.lmask) == op.lmask) v[r] = cmul(v[r], f);
-        } break;
-        case TK_RZ: {
-          const double2 m00 = make_double2(op.c[0], op.c[1]);
-          const double2 m11 = make_double2(op.c[2], op.c[3]);
-#pragma unroll
-          for (int r = 0; r < kR; ++r)
-            v[r] = (((thr_l | regl[r]) >> op.t) & 1u) ? cmul(v[r], m11) : cmul(v[r], m00);
-        } break;
-        case TK_SWAP: {
-          const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
-          const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
-          switch (a0 * 4 + b0) {
-            case 1: reg_swap<0, 1>(v, op, thr_l, regl); break;
-            case 2: reg_swap<0, 2>(v, op, thr_l, regl); break;
-            case 3: reg_swap<0, 3>(v, op, thr_l, regl); break;
-            case 6: reg_swap<1, 2>(v, op, thr_l, regl); break;
-            case 7: reg_swap<1, 3>(v, op, thr_l, regl); break;
-            default: reg_swap<2, 3>(v, op, thr_l, regl); break;
-          }
-        } break;
-        case TK_MAT4: {
-          const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
-          const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
-          const double* u = pool + op.aux;
-          switch (a0 * 4 + b0) {
-            case 1: reg_mat4<0, 1>(v, u); break;
-            case 2: reg_mat4<0, 2>(v, u); break;
-            case 3: reg_mat4<0, 3>(v, u); break;
-            case 6: reg_mat4<1, 2>(v, u); break;
-            case 7: reg_mat4<1, 3>(v, u); break;
-            default: reg_mat4<2, 3>(v, u); break;
-          }
-        } break;
-        default:
-          break;
-      }
+      for (int j = 0; j < kRegBits; ++j) rpos[j] = op.ins[j];
+      set_layout(L, tid, k, rpos);
+      if (act) lds_to_regs(v, L, lds);
+      __syncthreads();
+    } else if ((((uint64_t)base ^ frame) & op.gmask) == op.gval) {  // uniform over the tile
+      if ((L.thr_l & op.tl) == op.tl) reg_op<F>(v, op, L, pool);
     }
     if (nz.on && op.nq) {  // trajectory errors after the gate, operand order
       const double p = op.psel == 1 ? nz.p1 : nz.p2;
@@ -380,7 +453,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         for (int s = 0; s < k; ++s)
           if (geo.spos[s] == q) lb = s;
         if (lb >= 0) {
-          reg_pauli(v, code, (uint32_t)lb, rpos, thr_l, regl, lds, act);
+          reg_pauli(v, code, (uint32_t)lb, L, lds, act);
         } else if (code == 3) {  // Z on an outside qubit: a tile-wide sign
           if ((((uint64_t)base ^ frame) >> q) & 1ULL) {
 #pragma unroll
@@ -400,7 +473,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
   }
   if (act) {
 #pragma unroll
-    for (int r = 0; r < kR; ++r) a[base | gthr | scatter<I>(regl[r], geo)] = v[r];
+    for (int r = 0; r < kR; ++r) a[gaddr(r)] = v[r];
   }
   return frame;
 }

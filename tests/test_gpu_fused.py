@@ -11,6 +11,7 @@ from qoa_amd import workloads as W
 from qoa_amd._native import QOA_CZ_REF_MASKS
 
 QOA_FUSE_LDS_ONLY = 2
+QOA_FUSE_FMA = 4
 MODES = [0, QOA_FUSE_LDS_ONLY]  # register-resident segments (default) / LDS-resident
 from test_planner import mixed_circuit
 
@@ -58,12 +59,13 @@ def test_mixed_gate_surface_fused(n, tile, flags):
 @pytest.mark.parametrize("n", [1, 2, 5, 8, 9, 12, 13])
 def test_whole_state_single_tile(n):
     g = W.layered_random(n, 12, seed=n, ladders=True)
-    order, passes = plan_fused(n, g)
+    tile = 13 if n == 13 else 0  # the default tile holds 12 qubits
+    order, passes = plan_fused(n, g, tile_qubits=tile)
     assert passes == 1
     ref = po.state(n)
     po.run_gates(ref, g, order=order)
     with StateVector(n) as sv:
-        sv.run_fused(g)
+        sv.run_fused(g, tile_qubits=tile)
         assert np.array_equal(sv.download(), ref)
 
 
@@ -94,7 +96,27 @@ def test_index64_paths_match():
     assert outs[0][2] == outs[1][2] and outs[0][3] == outs[1][3]
 
 
-@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("tile", [6, 8, 12])
+def test_fma_mode_within_tolerance(tile):
+    """FMA products: not bit-identical, but every amplitude within 1e-12 of
+    the oracle (original gate order) and fidelity >= 1 - 1e-12, on the
+    layered circuit and on the full gate surface (2x2, 4x4, controls)."""
+    n = 16
+    g = W.layered_random(n, 20, seed=2, ladders=True)
+    gm, mats = mixed_circuit(n, 400, seed=tile)
+    for gates, m in ((g, None), (gm, mats)):
+        a0 = random_state(n, seed=tile)
+        ref = a0.copy()
+        po.run_gates(ref, gates, matrices=m)
+        with StateVector(n) as sv:
+            sv.upload(a0)
+            sv.run_fused(gates, tile_qubits=tile, flags=QOA_FUSE_FMA, matrices=m)
+            out = sv.download()
+        assert np.max(np.abs(out - ref)) <= 1e-12
+        assert fidelity(out, ref) >= 1 - 1e-12
+
+
+@pytest.mark.parametrize("mode", MODES + [QOA_FUSE_FMA])
 def test_c2_28q_fused_vs_per_gate_full_size(mode):
     """The full 28-qubit, 3164-gate circuit: fused passes against per-gate
     sweeps on the device (size-independent checks: unit norm, agreement

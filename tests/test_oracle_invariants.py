@@ -155,3 +155,37 @@ def test_depolarizing_channel_statistics():
     expect = [math.comb(n, j) * pf ** j * (1 - pf) ** (n - j) * trials for j in range(n + 1)]
     for o, e in zip(errs, expect):
         assert abs(o - e) < 5 * math.sqrt(e) + 5
+
+
+@pytest.mark.parametrize("n", [1, 5, 13, 14])
+def test_sampler_inverts_the_cdf(n):
+    """orc_sample returns the index whose CDF interval holds u*total (checked
+    against numpy's cumulative sum away from interval boundaries), never an
+    index of probability zero, and the last nonzero index for u -> 1."""
+    a = random_state(n, seed=n)
+    a[::3] = 0  # zero-probability entries are never chosen
+    p = np.abs(a) ** 2
+    cdf = np.cumsum(p)
+    L = po.lib()
+    rng = np.random.default_rng(n)
+    for u in rng.random(200):
+        t = u * cdf[-1]
+        want = int(np.searchsorted(cdf, t, side="right"))
+        got = L.orc_sample(po._p(a), a.size, float(u))
+        if np.min(np.abs(cdf - t)) > 1e-12:
+            assert got == want, (u, got, want)
+        assert p[got] > 0
+    last_nz = int(np.nonzero(p)[0][-1])
+    assert L.orc_sample(po._p(a), a.size, 1.0 - 2.0 ** -53) == last_nz
+    assert L.orc_sample(po._p(a), a.size, 0.0) == int(np.nonzero(p)[0][0])
+
+
+def test_sample_uniforms_are_distinct_streams():
+    """The sampler's uniforms lie in [0,1), differ across shots, trajectories
+    and seeds, and have the mean and variance of U(0,1)."""
+    L = po.lib()
+    u = np.array([L.orc_sample_uniform(5, 0, s) for s in range(4000)])
+    assert np.all((u >= 0) & (u < 1)) and len(set(u.tolist())) == u.size
+    assert abs(u.mean() - 0.5) < 0.02 and abs(u.var() - 1 / 12) < 0.01
+    assert L.orc_sample_uniform(5, 1, 0) != L.orc_sample_uniform(5, 0, 0)
+    assert L.orc_sample_uniform(6, 0, 0) != L.orc_sample_uniform(5, 0, 0)

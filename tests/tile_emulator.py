@@ -20,7 +20,8 @@ from qoa_amd.state import gates_array
 TILE_OP = np.dtype([("kind", "u1"), ("t", "u1"), ("t2", "u1"), ("nins", "u1"), ("ins", "u1", 4),
                     ("lmask", "<u4"), ("aux", "<u4"), ("gmask", "<u8"), ("gval", "<u8"),
                     ("c", "<f8", 8), ("gi", "<u4"), ("nq", "u1"), ("psel", "u1"),
-                    ("qphys", "u1", 3), ("pad", "u1", 7)])
+                    ("qphys", "u1", 3), ("rsel", "u1"), ("rt", "u1"), ("outside", "u1"),
+                    ("tl", "<u4")])
 TK_H, TK_M, TK_X, TK_Y, TK_DNEG, TK_DPHASE, TK_RZ, TK_SWAP, TK_MAT4, TK_LAYOUT = range(10)
 
 
@@ -57,6 +58,19 @@ def _deposit(x: int, positions) -> int:
     return x
 
 
+def _check_reg_fields(op, kind: int, lm: int, layout) -> None:
+    """Register mode splits the controls by the current layout: rsel has slot
+    j set iff local bit layout[j] is a control, tl holds the other controls,
+    rt is the register slot of an RZ target (0xFF for a thread bit)."""
+    R = sum(1 << b for b in layout)
+    rsel = sum(1 << j for j, b in enumerate(layout) if lm >> b & 1)
+    assert int(op["rsel"]) == rsel and int(op["tl"]) == lm & ~R, "control split"
+    if kind == TK_RZ:
+        t = int(op["t"])
+        want = layout.index(t) if t in layout else 0xFF
+        assert int(op["rt"]) == want, "RZ slot"
+
+
 def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int) -> None:
     """Applies the program to the complex128 state `a` in place."""
     re, im = a.real.copy(), a.imag.copy()
@@ -78,9 +92,11 @@ def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int) -> None:
                 if kind == TK_LAYOUT:
                     layout = [int(x) for x in op["ins"]]
                     continue
+                lm = int(op["lmask"])
+                if reg_mode:
+                    _check_reg_fields(op, kind, lm, layout)
                 if (base & int(op["gmask"])) != int(op["gval"]):
                     continue
-                lm = int(op["lmask"])
                 c = op["c"]
                 if kind in (TK_H, TK_M, TK_X, TK_Y):
                     t = int(op["t"])
