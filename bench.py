@@ -183,12 +183,17 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--serve-rates", default="1000")
     ap.add_argument("--serve-jobs", type=int, default=1500)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only time the copy-like X sweep (reads and writes every amplitude "
+                         "once) used to scale HBM counter readings")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the state-vector path is "
               "single-device; multi-GPU sharding is a design note, not built)", file=sys.stderr)
         return 2
+    if args.calibrate:
+        return calibrate(args)
 
     from qoa_amd import StateVector
     from qoa_amd import workloads as W
@@ -275,6 +280,29 @@ def main() -> int:
         "final_norm_sqr": norm,
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def calibrate(args) -> int:
+    """X on the top qubit: a pure permutation sweep, 2^n * 16 B read and
+    written per launch (the copy kernel of known size)."""
+    from qoa_amd import StateVector
+
+    n = args.qubits
+    with StateVector(n) as sv:
+        sv.apply_reset_all()
+        sv.apply_x(n - 1)
+        sv.synchronize()
+        sv.timing_enable(True)
+        for _ in range(args.steps):
+            sv.apply_x(n - 1)
+        sv.synchronize()
+        kinds = kernel_table(sv)
+    perm = kinds["sweep_perm"]
+    print(json.dumps({"calibration": f"X sweep on qubit {n - 1} of a {n}-qubit complex128 state",
+                      "launches": perm["launches"], "avg_us": perm["avg_us"],
+                      "read_bytes_per_launch": 16 * 2**n, "write_bytes_per_launch": 16 * 2**n,
+                      "gbps": perm["gbps"]}), flush=True)
     return 0
 
 

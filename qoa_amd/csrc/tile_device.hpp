@@ -30,6 +30,11 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) {
 __device__ __forceinline__ double2 csub(double2 a, double2 b) {
   return make_double2(a.x - b.x, a.y - b.y);
 }
+// c ? a : b per component (a conditional on the vector type itself would
+// select addresses and keep the operands in memory)
+__device__ __forceinline__ double2 sel2(bool c, double2 a, double2 b) {
+  return make_double2(c ? a.x : b.x, c ? a.y : b.y);
+}
 
 // insert zero bits at the op's sorted local positions
 __device__ __forceinline__ uint32_t dep(uint32_t x, const TileOp& op) {
@@ -123,57 +128,156 @@ __device__ __forceinline__ void set_layout(RegLayout& L, uint32_t tid, int k, co
 
 constexpr int kHReal = 16;  // TK_H with a real factor: a' = (nf*Re, nf*Im)
 
+// Coefficients of a 2x2 op: c[0..3] = m00, m01, m10, m11 (TK_H: c[0] = nf).
+struct Coef4 {
+  double2 c[4];
+};
+
+template <int KIND, bool F>
+__device__ __forceinline__ void pair_math(double2 x, double2 y, const Coef4& k, double2& o0,
+                                          double2& o1) {
+  if (KIND == kHReal) {
+    // equals cmul((nf, 0), .) up to the sign of zero terms
+    const double nf = k.c[0].x;
+    o0 = make_double2(nf * (x.x + y.x), nf * (x.y + y.y));
+    o1 = make_double2(nf * (x.x - y.x), nf * (x.y - y.y));
+  } else if (KIND == TK_H) {
+    o0 = mulc<F>(k.c[0], cadd(x, y));
+    o1 = mulc<F>(k.c[0], csub(x, y));
+  } else if (KIND == TK_M) {
+    o0 = mul2c<F>(k.c[0], x, k.c[1], y);
+    o1 = mul2c<F>(k.c[2], x, k.c[3], y);
+  } else if (KIND == TK_X) {
+    o0 = y;
+    o1 = x;
+  } else {  // TK_Y
+    o0 = cmul(y, make_double2(0.0, -1.0));
+    o1 = cmul(x, make_double2(0.0, 1.0));
+  }
+}
+
+// 2x2 op on register slot J for the registers whose slots RS are all 1.
+template <int J, int RS, int KIND, bool F>
+__device__ __forceinline__ void reg_pair(double2 (&v)[kR], const Coef4& k) {
+#pragma unroll
+  for (int r = 0; r < kR; ++r) {
+    if (r & (1 << J)) continue;
+    if ((r & RS) != RS) continue;
+    const int r1 = r | (1 << J);
+    double2 o0, o1;
+    pair_math<KIND, F>(v[r], v[r1], k, o0, o1);
+    v[r] = o0;
+    v[r1] = o1;
+  }
+}
+
+// Register controls rsel picked at compile time (a chain of uniform tests).
+template <int J, int KIND, bool F, int RS = 0>
+__device__ __forceinline__ void reg_pair_rs(double2 (&v)[kR], const Coef4& k, uint32_t rsel) {
+  if constexpr (RS < kR) {
+    if constexpr ((RS & (1 << J)) == 0) {
+      if (rsel == (uint32_t)RS) {
+        reg_pair<J, RS, KIND, F>(v, k);
+        return;
+      }
+    }
+    reg_pair_rs<J, KIND, F, RS + 1>(v, k, rsel);
+  }
+}
+
+// Thread-bit controls: every lane computes, lanes whose control bits are
+// not all set keep their values (tok == false).
 template <int J, int KIND, bool F>
-__device__ __forceinline__ void reg_pair(double2 (&v)[kR], const TileOp& op) {
-  const uint32_t rsel = op.rsel;
-  const double2 c0 = make_double2(op.c[0], op.c[1]), c1 = make_double2(op.c[2], op.c[3]);
-  const double2 c2 = make_double2(op.c[4], op.c[5]), c3 = make_double2(op.c[6], op.c[7]);
+__device__ __forceinline__ void reg_pair_sel(double2 (&v)[kR], const Coef4& k, uint32_t rsel,
+                                             bool tok) {
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & (1 << J)) continue;
     if (((uint32_t)r & rsel) != rsel) continue;  // uniform
     const int r1 = r | (1 << J);
-    const double2 x = v[r], y = v[r1];
-    if (KIND == kHReal) {
-      // equals cmul((nf, 0), .) up to the sign of zero terms
-      const double nf = c0.x;
-      v[r] = make_double2(nf * (x.x + y.x), nf * (x.y + y.y));
-      v[r1] = make_double2(nf * (x.x - y.x), nf * (x.y - y.y));
-    } else if (KIND == TK_H) {
-      v[r] = mulc<F>(c0, cadd(x, y));
-      v[r1] = mulc<F>(c0, csub(x, y));
-    } else if (KIND == TK_M) {
-      v[r] = mul2c<F>(c0, x, c1, y);
-      v[r1] = mul2c<F>(c2, x, c3, y);
-    } else if (KIND == TK_X) {
-      v[r] = y;
-      v[r1] = x;
-    } else {  // TK_Y
-      v[r] = cmul(y, make_double2(0.0, -1.0));
-      v[r1] = cmul(x, make_double2(0.0, 1.0));
+    double2 o0, o1;
+    pair_math<KIND, F>(v[r], v[r1], k, o0, o1);
+    v[r] = sel2(tok, o0, v[r]);
+    v[r1] = sel2(tok, o1, v[r1]);
+  }
+}
+
+// Uncontrolled ops take the straight path; X (CNOT, Toffoli) with register
+// controls has its control sets compiled in; other controlled ops test the
+// register controls per pair.
+template <int KIND, bool F>
+__device__ __forceinline__ void reg_pair_any(double2 (&v)[kR], const Coef4& k, uint32_t j,
+                                             uint32_t rsel, uint32_t tl, bool tok) {
+  if (tl == 0 && rsel == 0) {
+    switch (j) {
+      case 0: reg_pair<0, 0, KIND, F>(v, k); break;
+      case 1: reg_pair<1, 0, KIND, F>(v, k); break;
+      case 2: reg_pair<2, 0, KIND, F>(v, k); break;
+      default: reg_pair<3, 0, KIND, F>(v, k); break;
+    }
+  } else if (KIND == TK_X && tl == 0) {
+    switch (j) {
+      case 0: reg_pair_rs<0, KIND, F>(v, k, rsel); break;
+      case 1: reg_pair_rs<1, KIND, F>(v, k, rsel); break;
+      case 2: reg_pair_rs<2, KIND, F>(v, k, rsel); break;
+      default: reg_pair_rs<3, KIND, F>(v, k, rsel); break;
+    }
+  } else {
+    switch (j) {
+      case 0: reg_pair_sel<0, KIND, F>(v, k, rsel, tok); break;
+      case 1: reg_pair_sel<1, KIND, F>(v, k, rsel, tok); break;
+      case 2: reg_pair_sel<2, KIND, F>(v, k, rsel, tok); break;
+      default: reg_pair_sel<3, KIND, F>(v, k, rsel, tok); break;
     }
   }
 }
 
-template <int KIND, bool F>
-__device__ __forceinline__ void reg_pair_j(double2 (&v)[kR], const TileOp& op) {
-  switch (op.ins[0]) {
-    case 0: reg_pair<0, KIND, F>(v, op); break;
-    case 1: reg_pair<1, KIND, F>(v, op); break;
-    case 2: reg_pair<2, KIND, F>(v, op); break;
-    default: reg_pair<3, KIND, F>(v, op); break;
+// Diagonal factor f on the registers whose slots RS are all 1.
+template <int RS, bool F>
+__device__ __forceinline__ void reg_phase(double2 (&v)[kR], double2 f) {
+#pragma unroll
+  for (int r = 0; r < kR; ++r)
+    if ((r & RS) == RS) v[r] = mulc<F>(v[r], f);
+}
+
+template <bool F, int RS = 0>
+__device__ __forceinline__ void reg_phase_rs(double2 (&v)[kR], double2 f, uint32_t rsel) {
+  if constexpr (RS < kR) {
+    if (rsel == (uint32_t)RS) {
+      reg_phase<RS, F>(v, f);
+      return;
+    }
+    reg_phase_rs<F, RS + 1>(v, f, rsel);
+  }
+}
+
+template <int RS>
+__device__ __forceinline__ void reg_scale(double2 (&v)[kR], double s) {
+#pragma unroll
+  for (int r = 0; r < kR; ++r)
+    if ((r & RS) == RS) v[r] = make_double2(v[r].x * s, v[r].y * s);
+}
+
+template <int RS = 0>
+__device__ __forceinline__ void reg_scale_rs(double2 (&v)[kR], double s, uint32_t rsel) {
+  if constexpr (RS < kR) {
+    if (rsel == (uint32_t)RS) {
+      reg_scale<RS>(v, s);
+      return;
+    }
+    reg_scale_rs<RS + 1>(v, s, rsel);
   }
 }
 
 template <int A, int B>
-__device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel) {
+__device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel, bool tok) {
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & ((1 << A) | (1 << B))) continue;
     if (((uint32_t)r & rsel) != rsel) continue;
-    const double2 u = v[r | (1 << B)];
-    v[r | (1 << B)] = v[r | (1 << A)];
-    v[r | (1 << A)] = u;
+    const double2 u = v[r | (1 << B)], w = v[r | (1 << A)];
+    v[r | (1 << B)] = sel2(tok, w, u);
+    v[r | (1 << A)] = sel2(tok, u, w);
   }
 }
 
@@ -297,67 +401,91 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
   if (act) lds_to_regs(v, L, lds);
   __syncthreads();
   if (code == 2) {
-    const double2 f = ((L.thr_l >> b) & 1u) ? make_double2(0.0, 1.0) : make_double2(0.0, -1.0);
+    const double2 f = make_double2(0.0, ((L.thr_l >> b) & 1u) ? 1.0 : -1.0);
 #pragma unroll
     for (int r = 0; r < kR; ++r) v[r] = cmul(v[r], f);
   }
 }
 
-// Applies one non-layout op whose tile condition holds to this thread's
-// registers (the thread-bit controls have been checked by the caller).
+// The byte fields of a TileOp, read as 32-bit words (scalar loads fetch
+// whole dwords; a byte read would become a vector load and a wait):
+// w0 = kind | t << 8 | t2 << 16 | nins << 24, w1 = ins[0..3],
+// w25 = nq | psel << 8 | qphys[0] << 16 | qphys[1] << 24,
+// w26 = qphys[2] | rsel << 8 | rt << 16 | outside << 24, w27 = tl.
+struct OpWords {
+  uint32_t w0, w1, w25, w26, w27;
+};
+static_assert(offsetof(TileOp, ins) == 4 && offsetof(TileOp, nq) == 100 &&
+                  offsetof(TileOp, qphys) == 102 && offsetof(TileOp, rsel) == 105 &&
+                  offsetof(TileOp, rt) == 106 && offsetof(TileOp, tl) == 108,
+              "TileOp word layout");
+
+__device__ __forceinline__ OpWords op_words(const TileOp& op) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(&op);
+  return {w[0], w[1], w[25], w[26], w[27]};
+}
+
+// Applies one non-layout op whose tile condition holds. tok: this thread's
+// control bits (tl) are all set; lanes without it keep their amplitudes
+// (for diagonal ops through a factor of exactly 1).
 template <bool F>
-__device__ __forceinline__ void reg_op(double2 (&v)[kR], const TileOp& op, const RegLayout& L,
+__device__ __forceinline__ void reg_op(double2 (&v)[kR], const TileOp& op, const OpWords& w,
+                                       bool tok, const RegLayout& L,
                                        const double* __restrict__ pool) {
-  switch (op.kind) {
-    case TK_H:
-      if (op.c[1] == 0.0) reg_pair_j<kHReal, F>(v, op);
-      else reg_pair_j<TK_H, F>(v, op);
-      break;
-    case TK_M: reg_pair_j<TK_M, F>(v, op); break;
-    case TK_X: reg_pair_j<TK_X, F>(v, op); break;
-    case TK_Y: reg_pair_j<TK_Y, F>(v, op); break;
-    case TK_DNEG: {
-      const uint32_t rsel = op.rsel;
+  const uint32_t kind = w.w0 & 0xFFu;
+  const uint32_t rsel = (w.w26 >> 8) & 0xFFu;
+  const uint32_t tl = w.w27;
+  const uint32_t s0 = w.w1 & 0xFFu, s1 = (w.w1 >> 8) & 0xFFu;
+  switch (kind) {
+    case TK_H: case TK_M: case TK_X: case TK_Y: {
+      Coef4 k;
 #pragma unroll
-      for (int r = 0; r < kR; ++r)
-        if (((uint32_t)r & rsel) == rsel) v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
+      for (int i = 0; i < 4; ++i) k.c[i] = make_double2(op.c[2 * i], op.c[2 * i + 1]);
+      if (kind == TK_H) {
+        if (op.c[1] == 0.0) reg_pair_any<kHReal, F>(v, k, s0, rsel, tl, tok);
+        else reg_pair_any<TK_H, F>(v, k, s0, rsel, tl, tok);
+      } else if (kind == TK_M) {
+        reg_pair_any<TK_M, F>(v, k, s0, rsel, tl, tok);
+      } else if (kind == TK_X) {
+        reg_pair_any<TK_X, F>(v, k, s0, rsel, tl, tok);
+      } else {
+        reg_pair_any<TK_Y, F>(v, k, s0, rsel, tl, tok);
+      }
     } break;
+    case TK_DNEG:
+      reg_scale_rs(v, tok ? -1.0 : 1.0, rsel);
+      break;
     case TK_DPHASE: {
-      const uint32_t rsel = op.rsel;
-      const double2 f = make_double2(op.c[0], op.c[1]);
-#pragma unroll
-      for (int r = 0; r < kR; ++r)
-        if (((uint32_t)r & rsel) == rsel) v[r] = mulc<F>(v[r], f);
+      const double2 f = sel2(tok, make_double2(op.c[0], op.c[1]), make_double2(1.0, 0.0));
+      reg_phase_rs<F>(v, f, rsel);
     } break;
     case TK_RZ: {
       const double2 m00 = make_double2(op.c[0], op.c[1]);
       const double2 m11 = make_double2(op.c[2], op.c[3]);
-      if (op.rt != 0xFF) {  // target held in register slot rt: the factor is uniform per r
-        const uint32_t j = op.rt;
+      const uint32_t rt = (w.w26 >> 16) & 0xFFu;
+      if (rt != 0xFFu) {  // target held in register slot rt: the factor is uniform per r
 #pragma unroll
-        for (int r = 0; r < kR; ++r) v[r] = mulc<F>(v[r], (((uint32_t)r >> j) & 1u) ? m11 : m00);
+        for (int r = 0; r < kR; ++r) v[r] = mulc<F>(v[r], sel2((((uint32_t)r >> rt) & 1u) != 0, m11, m00));
       } else {
-        const double2 f = ((L.thr_l >> op.t) & 1u) ? m11 : m00;
+        const uint32_t t = (w.w0 >> 8) & 0xFFu;
+        const double2 f = sel2(((L.thr_l >> t) & 1u) != 0, m11, m00);
 #pragma unroll
         for (int r = 0; r < kR; ++r) v[r] = mulc<F>(v[r], f);
       }
     } break;
     case TK_SWAP: {
-      const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
-      const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
-      const uint32_t rsel = op.rsel;
+      const uint32_t a0 = s0 < s1 ? s0 : s1, b0 = s0 < s1 ? s1 : s0;
       switch (a0 * 4 + b0) {
-        case 1: reg_swap<0, 1>(v, rsel); break;
-        case 2: reg_swap<0, 2>(v, rsel); break;
-        case 3: reg_swap<0, 3>(v, rsel); break;
-        case 6: reg_swap<1, 2>(v, rsel); break;
-        case 7: reg_swap<1, 3>(v, rsel); break;
-        default: reg_swap<2, 3>(v, rsel); break;
+        case 1: reg_swap<0, 1>(v, rsel, tok); break;
+        case 2: reg_swap<0, 2>(v, rsel, tok); break;
+        case 3: reg_swap<0, 3>(v, rsel, tok); break;
+        case 6: reg_swap<1, 2>(v, rsel, tok); break;
+        case 7: reg_swap<1, 3>(v, rsel, tok); break;
+        default: reg_swap<2, 3>(v, rsel, tok); break;
       }
     } break;
     case TK_MAT4: {
-      const int a0 = op.ins[0] < op.ins[1] ? op.ins[0] : op.ins[1];
-      const int b0 = op.ins[0] < op.ins[1] ? op.ins[1] : op.ins[0];
+      const uint32_t a0 = s0 < s1 ? s0 : s1, b0 = s0 < s1 ? s1 : s0;
       const double* u = pool + op.aux;
       switch (a0 * 4 + b0) {
         case 1: reg_mat4<0, 1, F>(v, u); break;
@@ -432,21 +560,23 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
 
   for (int o = 0; o < nops; ++o) {
     const TileOp& op = ops[o];
-    if (op.kind == TK_LAYOUT) {
+    const OpWords w = op_words(op);
+    if ((w.w0 & 0xFFu) == TK_LAYOUT) {
       if (act) regs_to_lds(v, L, lds);
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < kRegBits; ++j) rpos[j] = op.ins[j];
+      for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(w.w1 >> (8 * j));
       set_layout(L, tid, k, rpos);
       if (act) lds_to_regs(v, L, lds);
       __syncthreads();
     } else if ((((uint64_t)base ^ frame) & op.gmask) == op.gval) {  // uniform over the tile
-      if ((L.thr_l & op.tl) == op.tl) reg_op<F>(v, op, L, pool);
+      reg_op<F>(v, op, w, (L.thr_l & w.w27) == w.w27, L, pool);
     }
-    if (nz.on && op.nq) {  // trajectory errors after the gate, operand order
-      const double p = op.psel == 1 ? nz.p1 : nz.p2;
-      for (int j = 0; j < op.nq; ++j) {
-        const uint32_t q = op.qphys[j];
+    const uint32_t nq = w.w25 & 0xFFu;
+    if (nz.on && nq) {  // trajectory errors after the gate, operand order
+      const double p = ((w.w25 >> 8) & 0xFFu) == 1 ? nz.p1 : nz.p2;
+      for (uint32_t j = 0; j < nq; ++j) {
+        const uint32_t q = j == 0 ? (w.w25 >> 16) & 0xFFu : j == 1 ? w.w25 >> 24 : w.w26 & 0xFFu;
         const uint32_t code = traj_pauli(nz.seed, nz.traj, op.gi, q, p);
         if (!code) continue;
         int lb = -1;
@@ -462,8 +592,8 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         } else {
           frame ^= 1ULL << q;
           if (code == 2) {  // Y: phase -i / +i by the new logical bit
-            const double2 f = ((((uint64_t)base ^ frame) >> q) & 1ULL) ? make_double2(0.0, 1.0)
-                                                                        : make_double2(0.0, -1.0);
+            const double2 f =
+                make_double2(0.0, ((((uint64_t)base ^ frame) >> q) & 1ULL) ? 1.0 : -1.0);
 #pragma unroll
             for (int r = 0; r < kR; ++r) v[r] = cmul(v[r], f);
           }
