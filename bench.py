@@ -34,6 +34,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_TFLOPS = 157.3
 QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA = 2, 4
 
 
@@ -61,7 +62,8 @@ def cpu_baseline(n: int, gates: np.ndarray, sample_gates: int) -> dict:
 
 def kernel_table(sv) -> dict:
     kinds = {}
-    for kind in ("sweep_dense", "sweep_diag", "sweep_perm", "fused_pass", "reduce", "other"):
+    for kind in ("sweep_dense", "sweep_diag", "sweep_perm", "fused_pass", "fused_block",
+                 "reduce", "other"):
         ms, cnt, b = sv.timing_read(kind)
         if cnt:
             kinds[kind] = {"ms": ms, "launches": cnt, "bytes": b,
@@ -111,6 +113,42 @@ def extra_c1(args) -> dict:
             el, _ = run_circuit(sv, g, method, 20, 2, f"c1 {method}")
             out[f"{method}_us_per_circuit"] = el / 20 * 1e6
             out[f"{method}_circuits_per_s"] = 20 / el
+    return out
+
+
+def extra_c3(args) -> dict:
+    """30 qubits, depth 60 with CPHASE ladders: dense 5-qubit blocks on FP32
+    matrix cores (complex64, 8 GiB) against the unfused FP64 per-gate run
+    (complex128, 16 GiB), both resident, compared on the device."""
+    from qoa_amd import StateVector
+    from qoa_amd import workloads as W
+    from qoa_amd._native import QOA_C64
+
+    n, depth = 30, 60
+    g = W.layered_random(n, depth, seed=args.seed, ladders=True)
+    out = {"workload": f"{n}-qubit layered random circuit, depth {depth}, CPHASE ladders "
+                       f"({len(g)} gates)"}
+    with StateVector(n) as ref, StateVector(n, precision=QOA_C64) as s32:
+        t0 = time.perf_counter()
+        ref.run_gates(g)
+        ref.synchronize()
+        out["fp64_per_gate_s"] = time.perf_counter() - t0
+        log("c3: FP64 reference done")
+        s32.timing_enable(True)
+        t0 = time.perf_counter()
+        nb = s32.run_blocks(g)
+        s32.synchronize()
+        out["fp32_blocks_s"] = time.perf_counter() - t0
+        kk = kernel_table(s32)
+        d, f = ref.compare(s32)
+    blk = kk["fused_block"]
+    flops = float(2 ** (n + 8))
+    out.update({"blocks": nb, "gates_per_block": len(g) / nb, "max_abs_diff": d,
+                "one_minus_fidelity": 1.0 - f, "block_avg_us": blk["avg_us"],
+                "block_hbm_gbps": blk["gbps"], "block_hbm_frac": blk["gbps"] / HBM_PEAK_GBS,
+                "block_tflops": flops / (blk["avg_us"] * 1e-6) / 1e12,
+                "block_mfma_frac": flops / (blk["avg_us"] * 1e-6) / 1e12 / FP32_MFMA_TFLOPS,
+                "kernels": kk})
     return out
 
 
@@ -181,6 +219,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample-gates", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--extras", default="",
+                    help="comma list of extras to run (c1,c2,c3,c4,c5; default all)")
     ap.add_argument("--serve-rates", default="1000")
     ap.add_argument("--serve-jobs", type=int, default=1500)
     ap.add_argument("--calibrate", action="store_true",
@@ -215,8 +255,15 @@ def main() -> int:
     log(f"{args.method}: {elapsed / args.steps * 1e3:.1f} ms/step, {value:.3e} amp-updates/s, "
         f"dominant {dom[0]} {dom[1]['gbps']:.0f} GB/s, norm^2 {norm:.15f}")
 
+    want = set(x for x in args.extras.split(",") if x) if args.extras else None
+    if args.no_extras:
+        want = set()
+
+    def wanted(label: str) -> bool:
+        return want is None or label in want
+
     extras = {}
-    if not args.no_extras:
+    if wanted("c2"):
         for other in ("fused", "fused-fma", "per-gate", "fused-lds"):
             if other == args.method:
                 continue
@@ -227,14 +274,17 @@ def main() -> int:
     name = device_name(lib)
     sv.close()
 
-    if not args.no_extras:
-        for label, fn in (("c1", lambda: extra_c1(args)), ("c4_batch", lambda: extra_c4(args))):
-            try:
-                extras[label] = fn()
-                log(f"{label}: {json.dumps(extras[label])[:200]}")
-            except Exception as e:  # noqa: BLE001
-                extras[label] = {"error": str(e)}
-                log(f"{label} failed: {e}")
+    for label, fn in (("c1", lambda: extra_c1(args)), ("c3_blocks", lambda: extra_c3(args)),
+                      ("c4_batch", lambda: extra_c4(args))):
+        if not wanted(label.split("_")[0]):
+            continue
+        try:
+            extras[label] = fn()
+            log(f"{label}: {json.dumps(extras[label])[:200]}")
+        except Exception as e:  # noqa: BLE001
+            extras[label] = {"error": str(e)}
+            log(f"{label} failed: {e}")
+    if wanted("c5"):
         serve = []
         for r in [float(x) for x in args.serve_rates.split(",") if x]:
             try:

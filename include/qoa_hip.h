@@ -231,11 +231,39 @@ int qoa_plan_program(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
                      uint32_t max_ops, uint8_t* pass_spos_out, uint32_t* pass_range_out,
                      uint32_t max_passes, uint32_t* n_ops, uint32_t* n_passes, uint32_t* tile_k);
 
+/* Dense 5-qubit blocks on complex64 states (the block-fusion primitive the
+   boundary lists as apply_fused_block; no reference counterpart: the
+   reference applies every gate as its own FP64 sweep, vectorization.rs).
+   Gates are packed greedily into blocks acting on at most block_qubits
+   (<= 5) qubits; each block's 32x32 unitary is the FP64 product of its
+   gates rounded to FP32 and is applied with FP32 matrix cores. After the
+   last block the state is renormalised with an FP64 sum unless
+   QOA_BLOCK_NO_RENORM is set. MEASURE/RESET are rejected; the state must be
+   QOA_C64 with at least 5 qubits. */
+#define QOA_BLOCK_NO_RENORM 8u
+typedef struct qoa_block_opts {
+  uint32_t block_qubits;  /* 1..5 (0 = 5)                                */
+  uint32_t flags;         /* QOA_CZ_REF_MASKS, QOA_BLOCK_NO_RENORM         */
+} qoa_block_opts;
+int qoa_run_blocks(qoa_state* s, const qoa_gate* gates, uint32_t n_gates,
+                   const double* matrices, const qoa_block_opts* opts, uint32_t* n_blocks);
+/* Fuser only (host): per block the 5 qubits (ascending; local bit b of the
+   block index is qubit qubits[5*blk+b]) and the 32x32 FP64 unitary,
+   row-major, interleaved (re, im): 2048 doubles per block. NULL outputs
+   query the count. */
+int qoa_plan_blocks(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
+                    const double* matrices, const qoa_block_opts* opts, uint32_t* qubits_out,
+                    double* unitaries_out, uint32_t max_blocks, uint32_t* n_blocks);
+/* Compares two states of equal size and any precision on the device:
+   max |a_i - b_i| and fidelity |<a|b>|^2 / (<a|a><b|b>), FP64 sums. */
+int qoa_state_compare(qoa_state* a, qoa_state* b, double* max_abs_diff, double* fidelity);
+
 /* Kernel timing: when enabled, every kernel launched on the handle's stream
    is bracketed by HIP events; qoa_timing_read returns per-kind totals. */
 int qoa_timing_enable(qoa_state* s, int enable);
 /* kind names: "sweep_dense", "sweep_diag", "sweep_perm", "fused_pass",
-   "reduce", "other". Returns total ms and launch count for that kind. */
+   "reduce", "other", "fused_block". Returns total ms and launch count for
+   that kind. */
 int qoa_timing_read(qoa_state* s, const char* kind, double* total_ms,
                     uint64_t* launches, double* bytes);
 

@@ -5,7 +5,8 @@ include/qoa_hip.h); this package is the host-side mirror of the reference's
 state API plus the workload generators used by tests and benchmarks.
 """
 from ._native import GATE_DTYPE, OP, QoaError, load  # noqa: F401
-from .state import NoiseConfig, QuantumState, StateVector, gates_array, plan_fused  # noqa: F401
+from .state import (NoiseConfig, QuantumState, StateVector, gates_array, plan_blocks,  # noqa: F401
+                    plan_fused)
 
 __all__ = ["GATE_DTYPE", "OP", "QoaError", "load", "NoiseConfig", "QuantumState", "StateVector",
-           "gates_array", "plan_fused"]
+           "gates_array", "plan_blocks", "plan_fused"]

@@ -29,6 +29,7 @@ QOA_C64 = 1
 QOA_CZ_REF_MASKS = 1
 QOA_FUSE_LDS_ONLY = 2
 QOA_FUSE_FMA = 4
+QOA_BLOCK_NO_RENORM = 8
 
 NOISE_NONE, NOISE_IDEAL, NOISE_RANDOM, NOISE_FIXED = 0, 1, 2, 3
 
@@ -50,6 +51,10 @@ class QoaGate(C.Structure):
 class QoaFuseOpts(C.Structure):
     _fields_ = [("tile_qubits", C.c_uint32), ("low_qubits", C.c_uint32), ("flags", C.c_uint32),
                 ("reserved", C.c_uint32)]
+
+
+class QoaBlockOpts(C.Structure):
+    _fields_ = [("block_qubits", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class QoaJobDesc(C.Structure):
@@ -113,6 +118,10 @@ _SIGS = {
     "qoa_run_fused": ([P, P, U32, P, P], I32),
     "qoa_plan_fused": ([U32, P, U32, P, P, C.POINTER(U32)], I32),
     "qoa_tile_op_size": ([], I32),
+    "qoa_run_blocks": ([P, P, U32, P, C.POINTER(QoaBlockOpts), C.POINTER(U32)], I32),
+    "qoa_plan_blocks": ([U32, P, U32, P, C.POINTER(QoaBlockOpts), P, P, U32, C.POINTER(U32)],
+                        I32),
+    "qoa_state_compare": ([P, P, DP, DP], I32),
     "qoa_plan_program": ([U32, P, U32, P, P, P, U32, P, P, U32, C.POINTER(U32), C.POINTER(U32),
                           C.POINTER(U32)], I32),
     "qoa_timing_enable": ([P, I32], I32),

@@ -32,7 +32,8 @@ bool g_force_index64 = false;
 
 const char* kind_name(int k) {
   static const char* names[KK_COUNT] = {"sweep_dense", "sweep_diag", "sweep_perm",
-                                        "fused_pass",  "reduce",     "other"};
+                                        "fused_pass",  "reduce",     "other",
+                                        "fused_block"};
   return (k >= 0 && k < KK_COUNT) ? names[k] : "?";
 }
 
@@ -251,6 +252,7 @@ int qoa_state_create(uint32_t n_qubits, uint32_t precision, qoa_state** out) {
   s->amps = nullptr;
   s->red = nullptr;
   s->fused_cache = nullptr;
+  s->block_cache = nullptr;
   s->stream = nullptr;
   s->noise_kind = QOA_NOISE_NONE;
   s->noise_p = 0.0;
@@ -322,6 +324,7 @@ int qoa_state_destroy(qoa_state* s) {
   }
   for (auto e : s->event_pool) hipEventDestroy(e);
   fused_cache_free(s);
+  block_cache_free(s);
   if (s->amps) hipFree(s->amps);
   if (s->red) hipFree(s->red);
   if (s->stream) hipStreamDestroy(s->stream);

@@ -1,0 +1,573 @@
+// blocks.hip — dense 5-qubit blocks on complex64 states (FP32 MFMA, gfx950).
+//
+// Host: a greedy fuser packs gates into blocks that act on at most five
+// qubits: the block's qubit set grows from the first pending gate by the
+// qubit that lets it absorb the most gates, and a gate joins when its qubits
+// lie in the set and it commutes with every gate left for a later block
+// (they share no qubit that either mixes), so the emitted order is a valid
+// reordering. Each block's 32x32 unitary is the FP64 product of its
+// gates, rounded once to FP32.
+//
+// Device: the state viewed as 32 x 2^(n-5) (rows: the block's qubits,
+// columns: the other bits) is multiplied in place by U. The complex product
+// is the real GEMM [Yr; Yi] = [Ur -Ui; Ui Ur] [Xr; Xi] (M = K = 64) on
+// v_mfma_f32_32x32x2f32: one wave owns 32 columns; lane l holds column
+// l % 32 and the rows of parity l / 32, so its 16 complex amplitudes are
+// exactly the B operands of the 32 K-steps. Each workgroup stages the block's
+// A operands (16 KiB, laid out step-major so LDS reads are conflict-free).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "gates.hpp"
+#include "qoa_internal.hpp"
+
+namespace qoa {
+namespace {
+
+using cd = std::complex<double>;
+constexpr int kBlockQ = 5;
+constexpr int kDim = 1 << kBlockQ;         // 32
+constexpr int kWFloats = 64 * 64;          // A operands per block
+constexpr int kColGroups = 4;              // 32-column groups per wave
+constexpr int kBlockThreads = 256;         // 4 waves
+
+// ---- host fuser -------------------------------------------------------------
+struct Block {
+  uint8_t q[kBlockQ];              // physical qubits, ascending (local bit b = q[b])
+  std::vector<cd> u;               // 32x32 row-major, u[i*32+k]
+};
+
+// Applies f to the 32-vector v (local bits b <-> physical qubits q[b]).
+void apply_local(const FOp& f, const int* loc, cd* v) {
+  auto lm = [&](uint64_t m) {
+    uint32_t r = 0;
+    for (int q = 0; q < 64; ++q)
+      if ((m >> q) & 1ULL) r |= 1u << loc[q];
+    return r;
+  };
+  const uint32_t pm = lm(f.pmask);
+  const cd c0(f.c[0], f.c[1]), c1(f.c[2], f.c[3]), c2(f.c[4], f.c[5]), c3(f.c[6], f.c[7]);
+  switch (f.kind) {
+    case FK_H: case FK_M: case FK_X: case FK_Y: {
+      const uint32_t tb = 1u << loc[f.t];
+      for (uint32_t i = 0; i < (uint32_t)kDim; ++i) {
+        if ((i & tb) || (i & pm) != pm) continue;
+        const cd a = v[i], b = v[i | tb];
+        if (f.kind == FK_H) {
+          v[i] = c0 * (a + b);
+          v[i | tb] = c0 * (a - b);
+        } else if (f.kind == FK_M) {
+          v[i] = c0 * a + c1 * b;
+          v[i | tb] = c2 * a + c3 * b;
+        } else if (f.kind == FK_X) {
+          v[i] = b;
+          v[i | tb] = a;
+        } else {
+          v[i] = b * cd(0.0, -1.0);
+          v[i | tb] = a * cd(0.0, 1.0);
+        }
+      }
+    } break;
+    case FK_DNEG:
+      for (uint32_t i = 0; i < (uint32_t)kDim; ++i)
+        if ((i & pm) == pm) v[i] = -v[i];
+      break;
+    case FK_DPHASE:
+      for (uint32_t i = 0; i < (uint32_t)kDim; ++i)
+        if ((i & pm) == pm) v[i] *= c0;
+      break;
+    case FK_RZ: {
+      const uint32_t tb = 1u << loc[f.rzq];
+      for (uint32_t i = 0; i < (uint32_t)kDim; ++i) v[i] *= (i & tb) ? c1 : c0;
+    } break;
+    case FK_SWAP: {
+      const uint32_t m1 = 1u << loc[f.t], m2 = 1u << loc[f.t2];
+      for (uint32_t i = 0; i < (uint32_t)kDim; ++i) {
+        if ((i & (m1 | m2)) || (i & pm) != pm) continue;
+        std::swap(v[i | m1], v[i | m2]);
+      }
+    } break;
+    case FK_MAT4: {
+      // basis (t, t2) = (q1, q2) ordered by (higher, lower) physical qubit,
+      // as the per-gate 4x4 sweep
+      const int lo = std::min(loc[f.t], loc[f.t2]), hi = std::max(loc[f.t], loc[f.t2]);
+      const uint32_t ml = 1u << lo, mh = 1u << hi;
+      for (uint32_t i = 0; i < (uint32_t)kDim; ++i) {
+        if (i & (ml | mh)) continue;
+        const uint32_t idx[4] = {i, i | ml, i | mh, i | ml | mh};
+        cd w[4], o[4];
+        for (int q = 0; q < 4; ++q) w[q] = v[idx[q]];
+        for (int s = 0; s < 4; ++s) {
+          o[s] = 0.0;
+          for (int q = 0; q < 4; ++q) o[s] += cd(f.m4[8 * s + 2 * q], f.m4[8 * s + 2 * q + 1]) * w[q];
+        }
+        for (int s = 0; s < 4; ++s) v[idx[s]] = o[s];
+      }
+    } break;
+  }
+}
+
+int build_blocks(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats, uint32_t flags,
+                 uint32_t bq, std::vector<Block>* out) {
+  if (n < (uint32_t)kBlockQ || n > 36)
+    return set_error(QOA_ERR_ARG, "dense blocks need 5..36 qubits (state has %u)", n);
+  if (bq == 0) bq = kBlockQ;
+  if (bq > (uint32_t)kBlockQ) return set_error(QOA_ERR_ARG, "block_qubits %u exceeds 5", bq);
+  std::vector<FOp> ops;
+  ops.reserve(ng);
+  for (uint32_t i = 0; i < ng; ++i) {
+    const uint32_t op = g[i].op;
+    if (op == QOA_OP_MEASURE || op == QOA_OP_RESET || op == QOA_OP_RESET_ALL)
+      return set_error(QOA_ERR_UNSUPPORTED, "gate %u (op 0x%02x) is not unitary", i, op);
+    int rc = canonical_gate(n, g[i], mats, flags, &ops, i);
+    if (rc) return rc;
+  }
+  for (const FOp& f : ops)
+    if ((uint32_t)__builtin_popcountll(f.all) > bq)
+      return set_error(QOA_ERR_UNSUPPORTED, "gate %u touches more than %u qubits", f.gi, bq);
+  std::vector<uint32_t> rem(ops.size());
+  for (uint32_t i = 0; i < rem.size(); ++i) rem[i] = i;
+  // gates a block over qubit set S would take (scan stops once S is blocked)
+  // (a gate passes a gate left for later when they share no qubit that
+  // either of them mixes: then they commute)
+  struct Blk {
+    uint64_t any = 0, mix = 0;
+    bool free(const FOp& f) const { return (f.mixb & any) == 0 && ((f.all & ~f.mixb) & mix) == 0; }
+    void add(const FOp& f) {
+      any |= f.all;
+      mix |= f.mixb;
+    }
+  };
+  auto count = [&](uint64_t S) {
+    Blk b;
+    int cnt = 0;
+    for (uint32_t i : rem) {
+      const FOp& f = ops[i];
+      if (b.free(f) && (f.all & ~S) == 0) {
+        ++cnt;
+      } else {
+        b.add(f);
+        if ((b.mix & S) == S) break;
+      }
+    }
+    return cnt;
+  };
+  out->clear();
+  while (!rem.empty()) {
+    // grow the qubit set from the first pending gate, one qubit at a time,
+    // taking the candidate (a qubit of the next pending gates) that lets the
+    // block absorb the most gates
+    uint64_t Q = ops[rem[0]].all;
+    while ((uint32_t)__builtin_popcountll(Q) < bq) {
+      uint64_t cand = 0;
+      for (size_t r = 0; r < rem.size() && r < 64; ++r) cand |= ops[rem[r]].all;
+      cand &= ~Q;
+      if (!cand) break;
+      int best_cnt = -1;
+      uint64_t best = 0;
+      for (int q = 0; q < 64; ++q) {
+        if (!((cand >> q) & 1ULL)) continue;
+        const int c = count(Q | (1ULL << q));
+        if (c > best_cnt) {
+          best_cnt = c;
+          best = 1ULL << q;
+        }
+      }
+      Q |= best;
+    }
+    Blk blk;
+    std::vector<uint32_t> take, next;
+    for (uint32_t i : rem) {
+      const FOp& f = ops[i];
+      if (blk.free(f) && (f.all & ~Q) == 0) {
+        take.push_back(i);
+      } else {
+        blk.add(f);
+        next.push_back(i);
+      }
+    }
+    // pad to five qubits with the lowest others (identity on them)
+    for (uint32_t q = 0; q < n && __builtin_popcountll(Q) < kBlockQ; ++q) Q |= 1ULL << q;
+    Block b;
+    int loc[64];
+    for (int q = 0; q < 64; ++q) loc[q] = -1;
+    int j = 0;
+    for (uint32_t q = 0; q < n; ++q)
+      if ((Q >> q) & 1ULL) {
+        b.q[j] = (uint8_t)q;
+        loc[q] = j++;
+      }
+    b.u.assign(kDim * kDim, cd(0.0, 0.0));
+    for (int i = 0; i < kDim; ++i) b.u[i * kDim + i] = 1.0;
+    cd col[kDim];
+    for (int k = 0; k < kDim; ++k) {
+      for (int i = 0; i < kDim; ++i) col[i] = b.u[i * kDim + k];
+      for (uint32_t t : take) apply_local(ops[t], loc, col);
+      for (int i = 0; i < kDim; ++i) b.u[i * kDim + k] = col[i];
+    }
+    out->push_back(std::move(b));
+    rem.swap(next);
+  }
+  return QOA_OK;
+}
+
+// A operands, step-major: w[s*64 + l] for K-step s of tile s / 32 and lane l
+// (row i = l % 32, K index 2*(s%32) + l/32 of [Ur -Ui] or [Ui Ur]).
+void pack_operands(const Block& b, float* w) {
+  for (int s = 0; s < 64; ++s) {
+    const int tile = s / 32, st = s % 32;
+    for (int l = 0; l < 64; ++l) {
+      const int i = l % 32, h = l / 32;
+      const int k = 2 * (st % 16) + h;  // column of U
+      const cd u = b.u[i * kDim + k];
+      float val;
+      if (tile == 0) val = st < 16 ? (float)u.real() : (float)(-u.imag());
+      else val = st < 16 ? (float)u.imag() : (float)u.real();
+      w[s * 64 + l] = val;
+    }
+  }
+}
+
+// ---- device -------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct BlockArgs {
+  uint64_t ncols;  // 2^(n-5)
+  uint8_t q[kBlockQ];
+};
+
+template <typename I>
+__device__ __forceinline__ I deposit5(I x, const BlockArgs& ba) {
+#pragma unroll
+  for (int j = 0; j < kBlockQ; ++j) {
+    const int b = ba.q[j];
+    x = ((x >> b) << (b + 1)) | (x & ((I(1) << b) - I(1)));
+  }
+  return x;
+}
+
+template <typename I>
+__device__ __forceinline__ I rowoff(uint32_t r, const BlockArgs& ba) {
+  I o = 0;
+#pragma unroll
+  for (int j = 0; j < kBlockQ; ++j) o |= (I)((r >> j) & 1u) << ba.q[j];
+  return o;
+}
+
+template <typename I>
+__global__ __launch_bounds__(kBlockThreads) void k_block5(float2* __restrict__ a,
+                                                         const float* __restrict__ w,
+                                                         BlockArgs ba) {
+  __shared__ float sw[kWFloats];
+  for (int i = threadIdx.x; i < kWFloats / 4; i += kBlockThreads)
+    reinterpret_cast<float4*>(sw)[i] = reinterpret_cast<const float4*>(w)[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, j = lane & 31;
+  const uint64_t wave_col0 =
+      ((uint64_t)blockIdx.x * (kBlockThreads / 64) + wave) * (uint64_t)(32 * kColGroups);
+  // the column groups of this wave, loads of group g+1 issued before the
+  // matrix products of group g
+  auto load_group = [&](int g, float2 (&x)[16], I& base, bool& ok) {
+    const uint64_t c = wave_col0 + (uint64_t)g * 32 + j;
+    ok = c < ba.ncols;
+    base = deposit5<I>((I)(ok ? c : 0), ba);
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      x[s] = ok ? a[base | rowoff<I>((uint32_t)(2 * s + h), ba)] : make_float2(0.f, 0.f);
+  };
+  if (wave_col0 >= ba.ncols) return;  // uniform over the wave (no barrier follows)
+  float2 cur[16], nxt[16];
+  I base, nbase = 0;
+  bool ok, nok = false;
+  load_group(0, cur, base, ok);
+  for (int g = 0; g < kColGroups; ++g) {
+    const bool more = g + 1 < kColGroups && wave_col0 + (uint64_t)(g + 1) * 32 < ba.ncols;
+    if (more) load_group(g + 1, nxt, nbase, nok);
+    f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+    for (int s = 0; s < 32; ++s) {
+      const float b = s < 16 ? cur[s].x : cur[s - 16].y;
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(sw[s * 64 + lane], b, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(sw[(32 + s) * 64 + lane], b, acc1, 0, 0, 0);
+    }
+    if (ok) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        a[base | rowoff<I>((uint32_t)((r & 3) + 8 * (r >> 2) + 4 * h), ba)] =
+            make_float2(acc0[r], acc1[r]);
+    }
+    if (!more) break;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) cur[s] = nxt[s];
+    base = nbase;
+    ok = nok;
+  }
+}
+
+// Per-block partial sums of |a|^2 (fixed grid, fixed order).
+__global__ __launch_bounds__(256) void k_c64_sumsq(const float2* __restrict__ a, uint64_t len,
+                                                   double* __restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * 256) {
+    const float2 x = a[i];
+    s += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_c64_scale(float2* __restrict__ a, uint64_t len,
+                                                   double f) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * 256) {
+    const float2 x = a[i];
+    a[i] = make_float2((float)((double)x.x * f), (float)((double)x.y * f));
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ double2 load_cd(const void* p, uint64_t i) {
+  if constexpr (sizeof(T) == 16) {
+    return reinterpret_cast<const double2*>(p)[i];
+  } else {
+    const float2 x = reinterpret_cast<const float2*>(p)[i];
+    return make_double2((double)x.x, (double)x.y);
+  }
+}
+
+// Partials per block: Re, Im of sum conj(a) b, sum |a|^2, sum |b|^2, max |a-b|.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_compare(const void* __restrict__ a,
+                                                 const void* __restrict__ b, uint64_t len,
+                                                 double* __restrict__ part) {
+  __shared__ double red[5][256];
+  double re = 0, im = 0, na = 0, nb = 0, mx = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < len;
+       i += (uint64_t)gridDim.x * 256) {
+    const double2 x = load_cd<TA>(a, i), y = load_cd<TB>(b, i);
+    re += x.x * y.x + x.y * y.y;
+    im += x.x * y.y - x.y * y.x;
+    na += x.x * x.x + x.y * x.y;
+    nb += y.x * y.x + y.y * y.y;
+    const double dx = x.x - y.x, dy = x.y - y.y;
+    mx = fmax(mx, sqrt(dx * dx + dy * dy));
+  }
+  const int t = threadIdx.x;
+  red[0][t] = re;
+  red[1][t] = im;
+  red[2][t] = na;
+  red[3][t] = nb;
+  red[4][t] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      for (int q = 0; q < 4; ++q) red[q][t] += red[q][t + w];
+      red[4][t] = fmax(red[4][t], red[4][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t < 5) part[5 * blockIdx.x + t] = red[t][0];
+}
+
+constexpr int kRedGrid = 1024;
+
+struct BlockCache {
+  uint64_t key = 0;
+  std::vector<Block> blocks;
+  float* d_w = nullptr;
+};
+
+uint64_t fnv(const void* p, size_t n, uint64_t h) {
+  const unsigned char* c = (const unsigned char*)p;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= c[i];
+    h *= 0x100000001B3ULL;
+  }
+  return h;
+}
+
+int renormalize_c64(qoa_state* s) {
+  double* part = nullptr;
+  if (hipMalloc(&part, sizeof(double) * kRedGrid) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(QOA_ERR_OOM, "reduction buffer allocation failed");
+  }
+  std::vector<double> h(kRedGrid);
+  int rc = QOA_OK;
+  timing_begin(s, KK_REDUCE, 8.0 * (double)s->len);
+  hipLaunchKernelGGL(k_c64_sumsq, dim3(kRedGrid), dim3(256), 0, s->stream,
+                     (const float2*)s->amps, s->len, part);
+  timing_end(s);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(h.data(), part, sizeof(double) * kRedGrid, hipMemcpyDeviceToHost,
+                     s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess)
+    rc = set_error(QOA_ERR_HIP, "norm reduction failed");
+  hipFree(part);
+  if (rc) return rc;
+  double sum = 0.0;
+  for (double x : h) sum += x;
+  if (!(sum > 1e-12)) return set_error(QOA_ERR_ARG, "state norm %g is too small to renormalise", sum);
+  timing_begin(s, KK_REDUCE, 16.0 * (double)s->len);
+  hipLaunchKernelGGL(k_c64_scale, dim3(kRedGrid), dim3(256), 0, s->stream, (float2*)s->amps,
+                     s->len, 1.0 / std::sqrt(sum));
+  timing_end(s);
+  QOA_HIP_CHECK(hipGetLastError());
+  return QOA_OK;
+}
+
+}  // namespace
+
+void block_cache_free(qoa_state* s) {
+  BlockCache* c = (BlockCache*)s->block_cache;
+  if (!c) return;
+  if (c->d_w) hipFree(c->d_w);
+  delete c;
+  s->block_cache = nullptr;
+}
+
+}  // namespace qoa
+
+using namespace qoa;
+
+extern "C" int qoa_plan_blocks(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
+                               const qoa_block_opts* opts, uint32_t* qubits_out,
+                               double* unitaries_out, uint32_t max_blocks,
+                               uint32_t* n_blocks_out) {
+  if (!g && ng) return set_error(QOA_ERR_ARG, "gates is NULL");
+  if (!n_blocks_out) return set_error(QOA_ERR_ARG, "n_blocks_out is NULL");
+  std::vector<Block> blocks;
+  int rc = build_blocks(n, g, ng, mats, opts ? opts->flags : 0, opts ? opts->block_qubits : 0,
+                        &blocks);
+  if (rc) return rc;
+  *n_blocks_out = (uint32_t)blocks.size();
+  if (!qubits_out && !unitaries_out) return QOA_OK;
+  if (blocks.size() > max_blocks)
+    return set_error(QOA_ERR_ARG, "%zu blocks do not fit in %u", blocks.size(), max_blocks);
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    if (qubits_out)
+      for (int j = 0; j < kBlockQ; ++j) qubits_out[kBlockQ * b + j] = blocks[b].q[j];
+    if (unitaries_out)
+      for (int e = 0; e < kDim * kDim; ++e) {
+        unitaries_out[2 * (kDim * kDim * b + e)] = blocks[b].u[e].real();
+        unitaries_out[2 * (kDim * kDim * b + e) + 1] = blocks[b].u[e].imag();
+      }
+  }
+  return QOA_OK;
+}
+
+extern "C" int qoa_run_blocks(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
+                              const qoa_block_opts* opts, uint32_t* n_blocks_out) {
+  if (!valid(s)) return set_error(QOA_ERR_HANDLE, "invalid handle");
+  if (s->precision != QOA_C64)
+    return set_error(QOA_ERR_UNSUPPORTED, "dense blocks run on QOA_C64 states");
+  if (!g && ng) return set_error(QOA_ERR_ARG, "gates is NULL");
+  const uint32_t flags = opts ? opts->flags : 0, bq = opts ? opts->block_qubits : 0;
+  uint64_t key = 0xCBF29CE484222325ULL;
+  key = fnv(&s->n, sizeof(s->n), key);
+  key = fnv(&flags, sizeof(flags), key);
+  key = fnv(&bq, sizeof(bq), key);
+  key = fnv(&ng, sizeof(ng), key);
+  if (ng) key = fnv(g, sizeof(qoa_gate) * ng, key);
+  for (uint32_t i = 0; i < ng; ++i) {
+    if (g[i].op == QOA_OP_U2 && mats) key = fnv(mats + 8 * g[i].aux, 64, key);
+    if (g[i].op == QOA_OP_U4 && mats) key = fnv(mats + 32 * g[i].aux, 256, key);
+  }
+  BlockCache* c = (BlockCache*)s->block_cache;
+  if (!c || c->key != key) {
+    block_cache_free(s);
+    c = new BlockCache();
+    s->block_cache = c;
+    int rc = build_blocks(s->n, g, ng, mats, flags, bq, &c->blocks);
+    if (rc) {
+      block_cache_free(s);
+      return rc;
+    }
+    std::vector<float> w((size_t)kWFloats * (c->blocks.size() + 1));
+    for (size_t b = 0; b < c->blocks.size(); ++b) pack_operands(c->blocks[b], &w[kWFloats * b]);
+    if (hipMalloc(&c->d_w, sizeof(float) * w.size()) != hipSuccess) {
+      (void)hipGetLastError();
+      block_cache_free(s);
+      return set_error(QOA_ERR_OOM, "block operand upload failed");
+    }
+    QOA_HIP_CHECK(hipMemcpyAsync(c->d_w, w.data(), sizeof(float) * w.size(),
+                                 hipMemcpyHostToDevice, s->stream));
+    QOA_HIP_CHECK(hipStreamSynchronize(s->stream));
+    c->key = key;
+  }
+  const uint64_t ncols = s->len >> kBlockQ;
+  const uint64_t cols_per_wg = (uint64_t)(kBlockThreads / 64) * 32 * kColGroups;
+  const uint64_t grid = (ncols + cols_per_wg - 1) / cols_per_wg;
+  if (grid > 0x7FFFFFFFULL) return set_error(QOA_ERR_ARG, "state too large for one launch");
+  for (size_t b = 0; b < c->blocks.size(); ++b) {
+    BlockArgs ba{};
+    ba.ncols = ncols;
+    std::memcpy(ba.q, c->blocks[b].q, kBlockQ);
+    const float* w = c->d_w + (size_t)kWFloats * b;
+    timing_begin(s, KK_BLOCK, 16.0 * (double)s->len);
+    if (use32(s))
+      hipLaunchKernelGGL(k_block5<uint32_t>, dim3((unsigned)grid), dim3(kBlockThreads), 0,
+                         s->stream, (float2*)s->amps, w, ba);
+    else
+      hipLaunchKernelGGL(k_block5<uint64_t>, dim3((unsigned)grid), dim3(kBlockThreads), 0,
+                         s->stream, (float2*)s->amps, w, ba);
+    timing_end(s);
+    QOA_HIP_CHECK(hipGetLastError());
+  }
+  if (n_blocks_out) *n_blocks_out = (uint32_t)c->blocks.size();
+  if (!(flags & QOA_BLOCK_NO_RENORM)) return renormalize_c64(s);
+  return QOA_OK;
+}
+
+extern "C" int qoa_state_compare(qoa_state* a, qoa_state* b, double* max_abs_diff,
+                                 double* fidelity) {
+  if (!valid(a) || !valid(b)) return set_error(QOA_ERR_HANDLE, "invalid handle");
+  if (a->n != b->n) return set_error(QOA_ERR_ARG, "qubit counts differ");
+  QOA_HIP_CHECK(hipStreamSynchronize(b->stream));
+  double* part = nullptr;
+  if (hipMalloc(&part, sizeof(double) * 5 * kRedGrid) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(QOA_ERR_OOM, "comparison buffer allocation failed");
+  }
+  const bool a128 = a->precision == QOA_C128, b128 = b->precision == QOA_C128;
+  const dim3 grid(kRedGrid), block(256);
+  if (a128 && b128)
+    hipLaunchKernelGGL((k_compare<double2, double2>), grid, block, 0, a->stream, a->amps, b->amps, a->len, part);
+  else if (a128)
+    hipLaunchKernelGGL((k_compare<double2, float2>), grid, block, 0, a->stream, a->amps, b->amps, a->len, part);
+  else if (b128)
+    hipLaunchKernelGGL((k_compare<float2, double2>), grid, block, 0, a->stream, a->amps, b->amps, a->len, part);
+  else
+    hipLaunchKernelGGL((k_compare<float2, float2>), grid, block, 0, a->stream, a->amps, b->amps, a->len, part);
+  std::vector<double> h(5 * kRedGrid);
+  int rc = QOA_OK;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                     a->stream) != hipSuccess ||
+      hipStreamSynchronize(a->stream) != hipSuccess)
+    rc = set_error(QOA_ERR_HIP, "comparison failed");
+  hipFree(part);
+  if (rc) return rc;
+  double re = 0, im = 0, na = 0, nb = 0, mx = 0;
+  for (int i = 0; i < kRedGrid; ++i) {
+    re += h[5 * i];
+    im += h[5 * i + 1];
+    na += h[5 * i + 2];
+    nb += h[5 * i + 3];
+    mx = std::max(mx, h[5 * i + 4]);
+  }
+  if (max_abs_diff) *max_abs_diff = mx;
+  if (fidelity) *fidelity = (na > 0 && nb > 0) ? (re * re + im * im) / (na * nb) : 0.0;
+  return QOA_OK;
+}

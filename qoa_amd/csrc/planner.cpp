@@ -7,8 +7,8 @@
 // pass costs one read and one write of the state.
 //
 // A gate can run in a pass when every qubit it mixes (targets of non-diagonal
-// actions) is in S and none of its qubits was blocked by an earlier gate left
-// for a later pass. Qubits a gate only reads (controls, diagonal phases) may
+// actions) is in S and it commutes with every earlier gate left for a later
+// pass (they share no qubit that either of them mixes). Qubits a gate only reads (controls, diagonal phases) may
 // lie outside S: their bit is constant inside a tile, so the gate becomes a
 // per-tile condition (CNOT with an outside control is "X or nothing"; a phase
 // on an outside qubit is a tile-wide scalar). Gates keep their original
@@ -20,25 +20,11 @@
 #include <cmath>
 #include <cstring>
 
+#include "gates.hpp"
 #include "qoa_internal.hpp"
 
 namespace qoa {
 namespace {
-
-enum FK { FK_H, FK_M, FK_X, FK_Y, FK_DNEG, FK_DPHASE, FK_RZ, FK_SWAP, FK_MAT4 };
-
-struct FOp {
-  int kind = 0;
-  int t = -1, t2 = -1;  // mixed (active) qubits
-  uint64_t pmask = 0;   // read-only qubits that must be 1
-  int rzq = -1;         // RZ: read-only qubit whose bit picks the factor
-  double c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const double* m4 = nullptr;
-  uint32_t gi = 0;
-  uint64_t active = 0, all = 0;
-  uint8_t nq = 0, psel = 1;  // trajectory-noise operands (the gate's qubits)
-  uint8_t qphys[3] = {0, 0, 0};
-};
 
 // Complex64::from_polar(1.0, th), cos/sin from one sincos call (as api.cpp)
 inline void polar(double th, double* out) {
@@ -159,6 +145,7 @@ int canonical(uint32_t n, const qoa_gate& g, const double* mats, uint32_t flags,
   }
   f.active = (f.t >= 0 ? bit(f.t) : 0) | (f.t2 >= 0 ? bit(f.t2) : 0);
   f.all = f.active | f.pmask | (f.rzq >= 0 ? bit(f.rzq) : 0);
+  f.mixb = f.active;
   // noise operands in gate order: one-qubit gates use p1, the others p2
   switch (g.op) {
     case QOA_OP_CNOT: case QOA_OP_CZ: case QOA_OP_CPHASE: case QOA_OP_SWAP:
@@ -174,18 +161,32 @@ int canonical(uint32_t n, const qoa_gate& g, const double* mats, uint32_t flags,
   return 0;
 }
 
+// Gates left for later: an action may run ahead of them when its mixed
+// qubits avoid every qubit they touch and its diagonal qubits avoid the
+// qubits they mix (then it commutes with each of them).
+template <typename M>
+struct BlockerT {
+  M any = 0, mix = 0;
+  bool free(M all, M mixb) const { return (mixb & any) == 0 && ((all & ~mixb) & mix) == 0; }
+  void add(M all, M mixb) {
+    any |= all;
+    mix |= mixb;
+  }
+};
+using Blocker = BlockerT<uint64_t>;
+
 // Number of ops of `rem` that a pass over S would run.
 int simulate(const std::vector<FOp>& ops, const std::vector<uint32_t>& rem, uint64_t S,
              uint64_t allq) {
-  uint64_t blocked = 0;
+  Blocker b;
   int cnt = 0;
   for (uint32_t i : rem) {
     const FOp& f = ops[i];
-    if ((f.all & blocked) == 0 && (f.active & ~S) == 0) {
+    if (b.free(f.all, f.mixb) && (f.active & ~S) == 0) {
       ++cnt;
     } else {
-      blocked |= f.all;
-      if ((blocked & allq) == allq) break;
+      b.add(f.all, f.mixb);
+      if ((b.mix & allq) == allq) break;
     }
   }
   return cnt;
@@ -294,38 +295,39 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
 // stays a valid reordering. The pass starts and ends in the layout that
 // holds the top kRegBits local bits in registers (coalesced HBM access).
 struct SegOp {
-  uint32_t act, all;
+  uint32_t act, all, mixb;
 };
 
 inline SegOp seg_masks_local(const TileOp& op) {
   switch (op.kind) {
     case TK_H: case TK_M: case TK_X: case TK_Y:
-      return {1u << op.t, (1u << op.t) | op.lmask};
+      return {1u << op.t, (1u << op.t) | op.lmask, 1u << op.t};
     case TK_SWAP: case TK_MAT4: {
       const uint32_t a = (1u << op.t) | (1u << op.t2);
-      return {a, a | op.lmask};
+      return {a, a | op.lmask, a};
     }
-    case TK_RZ: return {0u, 1u << op.t};
-    default: return {0u, op.lmask};
+    case TK_RZ: return {0u, 1u << op.t, 0u};
+    default: return {0u, op.lmask, 0u};
   }
 }
 
 inline SegOp seg_masks(const TileOp& op) {
   SegOp m = seg_masks_local(op);
   if (op.outside) m.all |= 1u << 31;  // ordered against other outside-qubit ops
+  if (op.nq || op.outside) m.mixb = m.all;  // trajectory noise follows: keep the order
   return m;
 }
 
 int seg_sim(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint32_t R,
             uint32_t allb) {
-  uint32_t blocked = 0;
+  BlockerT<uint32_t> b;
   int cnt = 0;
   for (uint32_t i : rem) {
-    if ((m[i].all & blocked) == 0 && (m[i].act & ~R) == 0) {
+    if (b.free(m[i].all, m[i].mixb) && (m[i].act & ~R) == 0) {
       ++cnt;
     } else {
-      blocked |= m[i].all;
-      if ((blocked & allb) == allb) break;
+      b.add(m[i].all, m[i].mixb);
+      if ((b.mix & allb) == allb) break;
     }
   }
   return cnt;
@@ -405,11 +407,11 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
       R = nr;
       out->push_back(layout_op(R));
     }
-    uint32_t blocked = 0;
+    BlockerT<uint32_t> blk;
     std::vector<uint32_t> next;
     next.reserve(rem.size());
     for (uint32_t i : rem) {
-      if ((m[i].all & blocked) == 0 && (m[i].act & ~R) == 0) {
+      if (blk.free(m[i].all, m[i].mixb) && (m[i].act & ~R) == 0) {
         TileOp op = ops[i];
         if (m[i].act) {
           op.ins[0] = slot_of(R, op.t);
@@ -422,7 +424,7 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
           last_gi = gis[i];
         }
       } else {
-        blocked |= m[i].all;
+        blk.add(m[i].all, m[i].mixb);
         next.push_back(i);
       }
     }
@@ -432,6 +434,11 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
 }
 
 }  // namespace
+
+int canonical_gate(uint32_t n, const qoa_gate& g, const double* mats, uint32_t flags,
+                   std::vector<FOp>* out, uint32_t gi) {
+  return canonical(n, g, mats, flags, out, gi);
+}
 
 int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
                const PlanOptions& opt, FusedPlan* plan) {
@@ -457,6 +464,8 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
     if (rc) return rc;
     if (ops.size() == before) plan->order.push_back(i);  // a no-op: position is irrelevant
   }
+  if (opt.flags & kPlanNoise)  // errors follow every gate: no gate passes another
+    for (FOp& f : ops) f.mixb = f.all;
   const uint64_t allq = n >= 64 ? ~0ULL : ((1ULL << n) - 1);
   const uint64_t low = m ? ((1ULL << m) - 1) : 0;
   std::vector<uint32_t> rem(ops.size());
@@ -513,19 +522,19 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
         ((k >= 8 || (opt.flags & kPlanForceReg)) && k >= (uint32_t)kRegBits &&
          !(opt.flags & QOA_FUSE_LDS_ONLY)) ? 1u : 0u;
     pp.op_begin = (uint32_t)plan->ops.size();
-    uint64_t blocked = 0;
+    Blocker blk;
     std::vector<uint32_t> next;
     next.reserve(rem.size());
     std::vector<TileOp> pass_ops;
     std::vector<uint32_t> pass_gi;
     for (uint32_t i : rem) {
       const FOp& f = ops[i];
-      if ((f.all & blocked) == 0 && (f.active & ~S) == 0) {
+      if (blk.free(f.all, f.mixb) && (f.active & ~S) == 0) {
         const size_t before = pass_ops.size();
         emit(f, S, pp.spos, (int)k, &pass_ops, &plan->pool, (opt.flags & kPlanNoise) != 0);
         for (size_t e = before; e < pass_ops.size(); ++e) pass_gi.push_back(f.gi);
       } else {
-        blocked |= f.all;
+        blk.add(f.all, f.mixb);
         next.push_back(i);
       }
     }

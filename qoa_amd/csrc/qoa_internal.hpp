@@ -29,7 +29,8 @@ enum KernelKind : int {
   KK_FUSED = 3,   // LDS-tiled multi-gate passes
   KK_REDUCE = 4,  // norm / probability reductions and collapses
   KK_OTHER = 5,
-  KK_COUNT = 6
+  KK_BLOCK = 6,   // dense 5-qubit blocks (complex64, MFMA)
+  KK_COUNT = 7
 };
 const char* kind_name(int k);
 
@@ -64,6 +65,7 @@ struct qoa_state {
   hipStream_t stream;
   double* red;         // reduction scratch: partials + result slots
   void* fused_cache;   // last fused plan and its device copy (fused.hip)
+  void* block_cache;   // last dense-block plan and its device tables (blocks.hip)
   // QuantumState-level fields (src/runtime/quantum_state.rs:37-53)
   int noise_kind;
   double noise_p;
@@ -130,6 +132,9 @@ int launch_sample(hipStream_t st, const double2* states, uint64_t stride, uint32
                   uint32_t ntraj, uint64_t seed, uint32_t count, const uint64_t* frames,
                   uint32_t frame_stride, double* cs, uint64_t* out);
 int launch_download_c64(qoa_state* s, double* dev_dst);
+
+// dense 5-qubit blocks (blocks.hip)
+void block_cache_free(qoa_state* s);
 
 // fused passes (fused.hip / planner.cpp)
 struct PlanOptions {

@@ -185,6 +185,24 @@ class StateVector:
         check(self._lib.qoa_run_fused(self._h, ptr(g), len(g), None if m is None else ptr(m),
                                       C.byref(o)))
 
+    def run_blocks(self, gates, block_qubits: int = 0, flags: int = 0,
+                   matrices: Optional[np.ndarray] = None) -> int:
+        """Dense 5-qubit blocks on FP32 matrix cores (complex64 states only);
+        returns the number of blocks."""
+        g = gates_array(gates)
+        m = None if matrices is None else np.ascontiguousarray(matrices, dtype=np.complex128)
+        o = N.QoaBlockOpts(block_qubits, flags)
+        nb = C.c_uint32()
+        check(self._lib.qoa_run_blocks(self._h, ptr(g), len(g), None if m is None else ptr(m),
+                                       C.byref(o), C.byref(nb)))
+        return int(nb.value)
+
+    def compare(self, other: "StateVector") -> tuple[float, float]:
+        """(max |a_i - b_i|, fidelity) computed on the device."""
+        d, f = C.c_double(), C.c_double()
+        check(self._lib.qoa_state_compare(self._h, other._h, C.byref(d), C.byref(f)))
+        return d.value, f.value
+
     # -- timing ------------------------------------------------------------------
     def timing_enable(self, on: bool = True) -> None:
         check(self._lib.qoa_timing_enable(self._h, 1 if on else 0))
@@ -206,6 +224,25 @@ def plan_fused(n_qubits: int, gates, tile_qubits: int = 0, low_qubits: int = 0,
     o = N.QoaFuseOpts(tile_qubits, low_qubits, flags, 0)
     check(lib.qoa_plan_fused(n_qubits, ptr(g), len(g), C.byref(o), ptr(order), C.byref(np_)))
     return order, int(np_.value)
+
+
+def plan_blocks(n_qubits: int, gates, block_qubits: int = 0, flags: int = 0,
+                matrices: Optional[np.ndarray] = None) -> tuple[np.ndarray, np.ndarray]:
+    """The dense-block fuser's output: qubits (blocks x 5, ascending) and the
+    FP64 32x32 unitaries (blocks x 32 x 32), in execution order."""
+    lib = load()
+    g = gates_array(gates)
+    m = None if matrices is None else np.ascontiguousarray(matrices, dtype=np.complex128)
+    o = N.QoaBlockOpts(block_qubits, flags)
+    nb = C.c_uint32()
+    mp = None if m is None else ptr(m)
+    check(lib.qoa_plan_blocks(n_qubits, ptr(g), len(g), mp, C.byref(o), None, None, 0,
+                              C.byref(nb)))
+    qs = np.zeros((nb.value, 5), dtype=np.uint32)
+    us = np.zeros((nb.value, 32, 32), dtype=np.complex128)
+    check(lib.qoa_plan_blocks(n_qubits, ptr(g), len(g), mp, C.byref(o), ptr(qs), ptr(us),
+                              nb.value, C.byref(nb)))
+    return qs, us
 
 
 class NoiseConfig:
