@@ -44,21 +44,46 @@ def log(msg: str) -> None:
 
 
 def cpu_baseline(n: int, gates: np.ndarray, sample_gates: int) -> dict:
-    """Times the CPU restatement (oracle, 1 thread) on the first gates of the
-    same circuit; the cost of a sweep does not depend on its position."""
+    """Times the CPU restatement of the reference loops on the first gates of
+    the same circuit (the cost of a sweep does not depend on its position):
+      strong   - every gate loop split over all host cores, no renormalisation
+                 (the headline value);
+      faithful - the reference's QuantumState model: one-thread sweep plus an
+                 all-core normalize after every gate
+                 (src/vectorization.rs:926-938, src/runtime/quantum_state.rs:868-900);
+      sweep_1t - the one-thread sweeps alone."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po  # the checker's CPU path, timed as the baseline
 
-    a = po.state(n)
-    sub = np.ascontiguousarray(gates[:sample_gates])
-    t0 = time.perf_counter()
-    po.run_gates(a, sub)
-    dt = time.perf_counter() - t0
-    rate = len(sub) * float(1 << n) / dt
-    return {"value": rate, "unit": "amp-updates/s", "cores": 1, "kind": "port",
-            "sample": f"first {len(sub)} gates of the {n}-qubit circuit on the CPU "
-                      f"restatement of src/vectorization.rs (1 thread, {dt:.1f} s), "
-                      f"extrapolated per gate; host has {os.cpu_count()} logical CPUs"}
+    cores = max(1, min(os.cpu_count() or 1, 16))
+
+    def timed(fn, count):
+        a = po.state(n)
+        sub = np.ascontiguousarray(gates[:count])
+        t0 = time.perf_counter()
+        fn(a, sub)
+        dt = time.perf_counter() - t0
+        return count * float(1 << n) / dt, dt
+
+    po.set_threads(cores)
+    try:
+        strong, t_strong = timed(po.run_gates, sample_gates * 8)
+    finally:
+        po.set_threads(1)
+    faithful, t_faith = timed(lambda a, s: po.run_gates_faithful(a, s, cores), sample_gates)
+    one, t_one = timed(po.run_gates, sample_gates)
+    return {"value": strong, "unit": "amp-updates/s", "cores": cores, "kind": "port",
+            "sample": f"first {sample_gates * 8} gates of the {n}-qubit circuit on the CPU "
+                      f"restatement of src/vectorization.rs with every gate loop split over "
+                      f"{cores} threads ({t_strong:.1f} s), extrapolated per gate; host has "
+                      f"{os.cpu_count()} logical CPUs",
+            "variants": {
+                "faithful": {"value": faithful, "threads": cores,
+                             "sample": f"first {sample_gates} gates, one-thread sweep + "
+                                       f"{cores}-thread normalize per gate ({t_faith:.1f} s)"},
+                "sweep_1thread": {"value": one, "threads": 1,
+                                  "sample": f"first {sample_gates} gates, one-thread sweeps "
+                                            f"({t_one:.1f} s)"}}}
 
 
 def kernel_table(sv) -> dict:

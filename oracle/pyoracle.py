@@ -47,12 +47,13 @@ _SIGS = {
     "orc_run_trajectory": [P, U64, P, U32, P, U32, U64, U32, D, D],
     "orc_run_trajectory_ordered": [P, U64, P, P, U32, P, U32, U64, U32, D, D],
     "orc_sample_uniform": [U64, U32, U32], "orc_sample": [P, U64, D],
+    "orc_set_threads": [I32], "orc_run_gates_faithful": [P, U64, P, U32, P, U32, I32],
 }
 _RES = {"orc_normalize": I32, "orc_measure": I32, "orc_run_gates": I32,
         "orc_run_gates_ordered": I32, "orc_splitmix64_next": U64, "orc_rng_u64": U64,
         "orc_rng_f64": D, "orc_norm_sqr": D, "orc_qs_apply": I32, "orc_qs_measure": I32,
         "orc_traj_pauli": U32, "orc_run_trajectory": I32, "orc_run_trajectory_ordered": I32,
-        "orc_sample_uniform": D, "orc_sample": U64}
+        "orc_sample_uniform": D, "orc_sample": U64, "orc_run_gates_faithful": I32}
 
 _lib = None
 
@@ -119,6 +120,21 @@ def run_gates(a: np.ndarray, gates: np.ndarray, flags: int = 0, matrices=None, o
         o = np.ascontiguousarray(order, dtype=np.uint32)
         rc = lib().orc_run_gates_ordered(_p(a), a.size, _p(g), _p(o), len(g), m, flags)
     if rc != 0:
+        raise ValueError("oracle rejected a gate")
+
+
+def set_threads(n: int) -> None:
+    """Threads for the gate loops (the all-core CPU baseline); 1 restores the
+    sequential restatement."""
+    lib().orc_set_threads(int(n))
+
+
+def run_gates_faithful(a: np.ndarray, gates: np.ndarray, threads: int, flags: int = 0,
+                       matrices=None) -> None:
+    """One-thread sweep plus a `threads`-thread normalize after every gate."""
+    g = np.ascontiguousarray(gates)
+    m = None if matrices is None else _p(np.ascontiguousarray(matrices, dtype=np.complex128))
+    if lib().orc_run_gates_faithful(_p(a), a.size, _p(g), len(g), m, flags, int(threads)) != 0:
         raise ValueError("oracle rejected a gate")
 
 

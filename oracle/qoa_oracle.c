@@ -23,6 +23,15 @@
 
 static inline uint64_t bit_of(uint64_t q) { return q < 64 ? (1ULL << q) : 0ULL; }
 
+/* Gate loops may be split over threads for the all-core CPU baseline
+   (orc_set_threads; 1 by default). Every iteration writes only its own
+   amplitude or pair, so results do not depend on the split; sums stay
+   sequential. */
+static int orc_threads = 1;
+void orc_set_threads(int t) { orc_threads = t < 1 ? 1 : t; }
+#define ORC_PAR_FOR \
+  _Pragma("omp parallel for schedule(static) if (orc_threads > 1) num_threads(orc_threads)")
+
 /* cos x and sin x from one glibc sincos call. The reference computes them
    side by side (Complex64::from_polar, the rx/ry set-up at
    src/vectorization.rs:529-535), which LLVM lowers to sincos on linux-gnu;
@@ -40,6 +49,7 @@ static inline void cmul(double xr, double xi, double yr, double yi, double* cr, 
 void orc_hadamard(double* a, uint64_t len, double nf_re, double nf_im, uint64_t mask_bit) {
   uint64_t bit = bit_of(mask_bit);
   if (!bit) return;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & bit) == 0) {
       uint64_t j = i | bit;
@@ -57,6 +67,7 @@ void orc_hadamard(double* a, uint64_t len, double nf_re, double nf_im, uint64_t 
 void orc_x(double* a, uint64_t len, uint64_t mask_bit) {
   uint64_t bit = bit_of(mask_bit);
   if (!bit) return;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & bit) == 0) {
       uint64_t j = i | bit;
@@ -73,6 +84,7 @@ void orc_x(double* a, uint64_t len, uint64_t mask_bit) {
 void orc_y(double* a, uint64_t len, uint64_t mask_bit) {
   uint64_t bit = bit_of(mask_bit);
   if (!bit) return;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & bit) == 0) {
       uint64_t j = i | bit;
@@ -89,6 +101,7 @@ void orc_y(double* a, uint64_t len, uint64_t mask_bit) {
 void orc_z(double* a, uint64_t len, uint64_t mask_bit) {
   uint64_t bit = bit_of(mask_bit);
   if (!bit) return;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & bit) != 0) {
       RE(a, i) = RE(a, i) * -1.0;
@@ -101,6 +114,7 @@ void orc_z(double* a, uint64_t len, uint64_t mask_bit) {
 void orc_phase_factor(double* a, uint64_t len, uint64_t mask_bit, double pf_re, double pf_im) {
   uint64_t bit = bit_of(mask_bit);
   if (!bit) return;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & bit) != 0) {
       double xr = RE(a, i), xi = IM(a, i);
@@ -156,6 +170,7 @@ void orc_reset(double* a, uint64_t len, uint64_t mask_bit) {
 /* apply_swap_simd, src/vectorization.rs:1176-1197 (masks built at :421-423). */
 void orc_swap(double* a, uint64_t len, uint64_t q1, uint64_t q2) {
   uint64_t m1 = bit_of(q1), m2 = bit_of(q2), comb = m1 | m2;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & comb) == 0) {
       uint64_t x = i | m2, y = i | m1;
@@ -171,6 +186,7 @@ void orc_swap(double* a, uint64_t len, uint64_t q1, uint64_t q2) {
 /* apply_controlled_swap_simd, src/vectorization.rs:1200-1232. */
 void orc_controlled_swap(double* a, uint64_t len, uint64_t c, uint64_t t1, uint64_t t2) {
   uint64_t cm = bit_of(c), m1 = bit_of(t1), m2 = bit_of(t2), comb = cm | m1 | m2;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & comb) == 0) {
       uint64_t x = i | cm | m2, y = i | cm | m1;
@@ -189,6 +205,7 @@ void orc_controlled_swap(double* a, uint64_t len, uint64_t c, uint64_t t1, uint6
 void orc_matrix2(double* a, uint64_t len, uint64_t mask_bit, const double* m) {
   uint64_t bit = bit_of(mask_bit);
   if (!bit) return;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & bit) == 0) {
       uint64_t j = i | bit;
@@ -232,6 +249,7 @@ void orc_rz(double* a, uint64_t len, uint64_t mask_bit, double angle) {
   cis(angle / 2.0, &c1, &s1);
   double m00r = 1.0 * c0, m00i = 1.0 * s0, m11r = 1.0 * c1, m11i = 1.0 * s1;
   uint64_t bit = bit_of(mask_bit);
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     double xr = RE(a, i), xi = IM(a, i);
     if ((i & bit) == 0)
@@ -246,6 +264,7 @@ void orc_rz(double* a, uint64_t len, uint64_t mask_bit, double angle) {
 void orc_cnot(double* a, uint64_t len, uint64_t control, uint64_t target) {
   uint64_t cm = bit_of(control), tm = bit_of(target);
   if (!cm || !tm) return;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & cm) == cm && (i & tm) == 0) {
       uint64_t j = i | tm;
@@ -261,6 +280,7 @@ void orc_cnot(double* a, uint64_t len, uint64_t control, uint64_t target) {
 /* apply_cz_simd, src/vectorization.rs:1163-1173 (takes MASKS). */
 void orc_cz(double* a, uint64_t len, uint64_t cmask, uint64_t tmask) {
   uint64_t m = cmask | tmask;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & m) == m) {
       RE(a, i) = RE(a, i) * -1.0;
@@ -276,6 +296,7 @@ void orc_cphase(double* a, uint64_t len, uint64_t cmask, uint64_t tmask, double 
   double c, s;
   cis(angle, &c, &s);
   double pr = 1.0 * c, pi = 1.0 * s;
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & m) == m) {
       double xr = RE(a, i), xi = IM(a, i);
@@ -303,6 +324,7 @@ void orc_custom_unitary(double* a, uint64_t len, uint64_t q, const double* u) {
 void orc_two_qubit_unitary(double* a, uint64_t len, uint64_t q1, uint64_t q2, const double* u) {
   uint64_t lo = q1 < q2 ? q1 : q2, hi = q1 < q2 ? q2 : q1;
   uint64_t ml = bit_of(lo), mh = bit_of(hi);
+  ORC_PAR_FOR
   for (uint64_t base = 0; base < len; ++base) {
     if (base & (ml | mh)) continue;
     uint64_t idx[4] = {base, base | ml, base | mh, base | ml | mh};
@@ -326,6 +348,7 @@ void orc_two_qubit_unitary(double* a, uint64_t len, uint64_t q1, uint64_t q2, co
    are set, exchange the target-0 and target-1 amplitudes. */
 void orc_toffoli(double* a, uint64_t len, uint64_t c1, uint64_t c2, uint64_t t) {
   uint64_t cm = bit_of(c1) | bit_of(c2), tm = bit_of(t);
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & cm) == cm && (i & tm) == 0) {
       uint64_t j = i | tm;
@@ -340,6 +363,7 @@ void orc_toffoli(double* a, uint64_t len, uint64_t c1, uint64_t c2, uint64_t t) 
    new[t=0] = (-i)*old[t=1], new[t=1] = (i)*old[t=0] where control set. */
 void orc_controlled_y(double* a, uint64_t len, uint64_t c, uint64_t t) {
   uint64_t cm = bit_of(c), tm = bit_of(t);
+  ORC_PAR_FOR
   for (uint64_t i = 0; i < len; ++i) {
     if ((i & cm) == cm && (i & tm) == 0) {
       uint64_t j = i | tm;
@@ -468,6 +492,40 @@ int orc_run_gates_ordered(double* a, uint64_t len, const qoa_gate* g, const uint
   for (uint32_t k = 0; k < n; ++k)
     if (apply_gate(a, len, &g[order[k]], mats, flags)) return -1;
   return 0;
+}
+
+/* The reference's QuantumState execution model as a timed baseline: each
+   gate is a one-thread sweep (src/vectorization.rs:926-938) followed by
+   normalize (src/runtime/quantum_state.rs:868, 888-907), whose sum and
+   division run on `threads` threads as rayon's par_iter does (:894-900).
+   The parallel sum's order differs from the sequential one at the ulp
+   level only. */
+int orc_run_gates_faithful(double* a, uint64_t len, const qoa_gate* g, uint32_t n,
+                           const double* mats, uint32_t flags, int threads) {
+  const int saved = orc_threads;
+  orc_threads = 1;
+  int rc = 0;
+  for (uint32_t k = 0; k < n && rc == 0; ++k) {
+    if (apply_gate(a, len, &g[k], mats, flags)) {
+      rc = -1;
+      break;
+    }
+    double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s) num_threads(threads)
+    for (uint64_t i = 0; i < len; ++i) s += RE(a, i) * RE(a, i) + IM(a, i) * IM(a, i);
+    if (s > 1e-12) {
+      const double nrm = sqrt(s);
+#pragma omp parallel for schedule(static) num_threads(threads)
+      for (uint64_t i = 0; i < len; ++i) {
+        RE(a, i) /= nrm;
+        IM(a, i) /= nrm;
+      }
+    } else {
+      orc_reset_all(a, len);
+    }
+  }
+  orc_threads = saved;
+  return rc;
 }
 
 /* ---- RNG ----------------------------------------------------------------- */

@@ -65,6 +65,12 @@ int orc_run_gates(double* a, uint64_t len, const qoa_gate* g, uint32_t n,
 /* Same gates applied in the order given by `order` (a planner's output). */
 int orc_run_gates_ordered(double* a, uint64_t len, const qoa_gate* g, const uint32_t* order,
                           uint32_t n, const double* matrices, uint32_t flags);
+/* Threads for the gate loops (CPU baseline "strong"; default 1). */
+void orc_set_threads(int threads);
+/* CPU baseline "faithful": one-thread sweep + `threads`-thread normalize
+   after every gate (the QuantumState execution model). */
+int orc_run_gates_faithful(double* a, uint64_t len, const qoa_gate* g, uint32_t n,
+                           const double* matrices, uint32_t flags, int threads);
 
 /* ---- counter-based RNG (Salmon et al., SC'11) and circuit PRNG ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -189,3 +189,23 @@ def test_sample_uniforms_are_distinct_streams():
     assert abs(u.mean() - 0.5) < 0.02 and abs(u.var() - 1 / 12) < 0.01
     assert L.orc_sample_uniform(5, 1, 0) != L.orc_sample_uniform(5, 0, 0)
     assert L.orc_sample_uniform(6, 0, 0) != L.orc_sample_uniform(5, 0, 0)
+
+
+def test_threaded_gate_loops_are_the_same_arithmetic():
+    """The all-core CPU baseline splits the gate loops over threads; every
+    iteration owns its amplitudes, so results stay bit-identical. The
+    faithful baseline (sweep + normalize per gate) equals the sequential run
+    normalised, to rounding."""
+    n = 12
+    g = W.layered_random(n, 10, seed=7, ladders=True)
+    a = random_state(n, seed=7)
+    seq, par, fai = a.copy(), a.copy(), a.copy()
+    po.run_gates(seq, g)
+    po.set_threads(4)
+    try:
+        po.run_gates(par, g)
+    finally:
+        po.set_threads(1)
+    assert np.array_equal(seq, par)
+    po.run_gates_faithful(fai, g, 4)
+    assert np.max(np.abs(fai - seq / np.linalg.norm(seq))) <= 1e-14
