@@ -234,7 +234,7 @@ _SCAN_Q = {0x04: 1, 0x05: 1, 0x06: 1, 0x07: 1, 0x0D: 1, 0x0E: 1, 0x0A: 1, 0x32: 
            0x0C: 1, 0x31: 1, 0x87: 1, 0x35: 2}
 
 
-def _length(payload: bytes, i: int) -> int:
+def _length(payload: bytes, i: int, skip_unknown: bool = False) -> int:
     op = payload[i]
     if op in _FIXED:
         return _FIXED[op]
@@ -243,19 +243,23 @@ def _length(payload: bytes, i: int) -> int:
         return (len(payload) if end < 0 else end + 1) - i
     if op == 0x9A:
         return 9 + struct.unpack("<Q", payload[i + 1:i + 9])[0]
+    if skip_unknown:  # the VM's handling of an opcode it does not know: one byte
+        return 1
     raise UnsupportedInstruction(f"opcode 0x{op:02X} at byte {i} is outside the gate-path subset")
 
 
-def scan_qubits(payload: bytes) -> int:
-    """The VM's first pass (src/main.rs:1880-2285): max operand byte + 1."""
+def scan_qubits(payload: bytes, skip_unknown: bool = False) -> int:
+    """The VM's first pass (src/main.rs:1880-2285): max operand byte + 1.
+    Opcodes outside the subset raise, or (skip_unknown) advance one byte as
+    the VM does for opcodes it does not know."""
     i, mx = 0, 0
     while i < len(payload):
         op = payload[i]
-        k = _SCAN_Q.get(op, 0)
+        k = _SCAN_Q.get(op, 0) if (op in _FIXED or op in (0x87, 0x9A)) else 0
         for j in range(k):
             if i + 1 + j < len(payload):
                 mx = max(mx, payload[i + 1 + j])
-        i += _length(payload, i)
+        i += _length(payload, i, skip_unknown)
     return 0 if (mx == 0 and not payload) else mx + 1
 
 
@@ -270,9 +274,13 @@ class RunResult:
 
 
 def execute(payload: bytes, backend, registers: Optional[list] = None,
-            echo: Callable[[str], None] = lambda s: None) -> RunResult:
+            echo: Callable[[str], None] = lambda s: None,
+            skip_unknown: bool = False) -> RunResult:
     """Runs `payload` on `backend` (methods apply(op_name, q0, q1, q2, theta),
-    measure(q) -> int raising ValueError for a bad qubit, attribute n)."""
+    measure(q) -> int raising ValueError for a bad qubit, attribute n).
+    An opcode outside the subset raises UnsupportedInstruction, or with
+    skip_unknown is logged and skipped one byte, as the VM skips opcodes it
+    does not know (the VM itself runs the ones it knows)."""
     res = RunResult(n_qubits=backend.n)
     if registers is not None:
         res.registers = list(registers)
@@ -285,7 +293,7 @@ def execute(payload: bytes, backend, registers: Optional[list] = None,
 
     while i < len(payload):
         op = payload[i]
-        step = _length(payload, i)
+        step = _length(payload, i, skip_unknown)
         if i + step > len(payload):
             res.log.append(f"incomplete instruction at byte {i}")
             break
@@ -361,6 +369,8 @@ def execute(payload: bytes, backend, registers: Optional[list] = None,
         elif op == 0xFF:
             res.halted = True
             break
+        elif op not in _FIXED:
+            res.log.append(f"unknown opcode 0x{op:02X} at byte {i}, skipping")
         # 0x04 0x0A 0x11 0x12 0x1D 0x24 0x01 0x74-0x77 0x79 0x7A 0x00 0x8D: read, ignored
         i += step
     return res
@@ -381,18 +391,18 @@ class HipBackend:
 
 
 def run_text(text: str, seed: Optional[int] = None, noise_config=None, strict: bool = True,
-             echo: Callable[[str], None] = lambda s: None):
+             echo: Callable[[str], None] = lambda s: None, skip_unknown: bool = False):
     """Compiles and runs a program on the device; returns (QuantumState,
     RunResult). The state has the VM's qubit count and uses the reference's
     CZ/CPHASE mask reading."""
     from .state import QuantumState
 
     payload = compile_text(text, strict=strict).payload
-    n = max(1, scan_qubits(payload))
+    n = max(1, scan_qubits(payload, skip_unknown))
     qs = QuantumState(n, noise_config=noise_config, cz_ref_masks=True)
     if seed is not None:
         qs.seed_rng(seed)
-    res = execute(payload, HipBackend(qs), echo=echo)
+    res = execute(payload, HipBackend(qs), echo=echo, skip_unknown=skip_unknown)
     return qs, res
 
 

@@ -36,6 +36,7 @@ PHASE 4 9.0          ; read and ignored by the VM
 RESET 3              ; read and ignored by the VM
 Z 2
 CHARLOAD 2 'Q'
+RSA                  ; swallowed by CHARLOAD's 4-byte advance in the VM
 CHAROUT 5
 MEAS 6
 VERBOSELOG 0 "done"

@@ -125,6 +125,20 @@ def test_charload_advance_quirk():
     assert b.qs.a[0] == 1 and np.count_nonzero(b.qs.a) == 1
 
 
+def test_misaligned_stream_raises_or_skips_unknown_opcodes():
+    """After CHARLOAD the VM reads CHAROUT's operand (0x01) as LOOPSTART,
+    which takes X's opcode as its operand, and X's operand (0x02) lands as an
+    opcode outside the subset: it raises by default and is skipped one byte
+    with skip_unknown (the VM's rule for opcodes it does not know)."""
+    payload = P.compile_text("CHARLOAD 0 'A'\nCHAROUT 1\nX 2\n").payload
+    assert payload == b"\x31\x00\x41\x18\x01\x07\x02"
+    with pytest.raises(P.UnsupportedInstruction):
+        P.scan_qubits(payload)
+    n = P.scan_qubits(payload, skip_unknown=True)
+    res = P.execute(payload, OracleBackend(n, 0), skip_unknown=True)
+    assert any("unknown opcode" in m for m in res.log)
+
+
 def test_loops_run_once_and_halt_stops():
     b = OracleBackend(1, 0)
     res = P.execute(P.compile_text("LOOPSTART 0\nX 0\nLOOPEND\nHALT\nX 0\n").payload, b)
