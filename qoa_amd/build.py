@@ -32,6 +32,12 @@ COMMON = [
     # switch arms that differ in register indices are not merged (merging
     # them leaves a dynamically indexed array in scratch memory).
     "-mllvm", "-simplifycfg-sink-common=false",
+    # Op dispatch in the register kernels branches on wave-uniform op
+    # words. Structurizing those uniform if/else arms into exec-masked flow
+    # blocks leaves phis the register coalescer cannot join, which costs a
+    # copy of every live amplitude register per op; uniform regions keep
+    # their plain scalar branches.
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall", "-Wno-unused-result", "-Wno-unused-command-line-argument",
     "-I", str(PKG.parent / "include"),
 ]

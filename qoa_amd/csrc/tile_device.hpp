@@ -133,6 +133,228 @@ struct Coef4 {
   double2 c[4];
 };
 
+// ---- in-place register arithmetic ------------------------------------------
+// Each primitive writes its results into the registers that held its inputs
+// ("+v" operands), so every amplitude keeps one register for the whole op
+// loop and the loop's branches merge without register copies (the compiler
+// otherwise places results in fresh registers and copies them back at every
+// merge). The instruction sequences are the num-complex expressions of the
+// per-gate sweeps, every product and sum rounded on its own; the *_fma forms
+// are the FMA arithmetic of mulc<true> / mul2c<true>.
+
+// (a, b) *= (c, d): (a c - b d, a d + b c)
+__device__ __forceinline__ void ip_cmul(double& a, double& b, double c, double d) {
+  double t1, t2, t3, t4;
+  asm("v_mul_f64 %[t1], %[a], %[c]\n\t"
+      "v_mul_f64 %[t2], %[b], %[d]\n\t"
+      "v_mul_f64 %[t3], %[a], %[d]\n\t"
+      "v_mul_f64 %[t4], %[b], %[c]\n\t"
+      "v_add_f64 %[a], %[t1], -%[t2]\n\t"
+      "v_add_f64 %[b], %[t3], %[t4]"
+      : [a] "+v"(a), [b] "+v"(b), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3),
+        [t4] "=&v"(t4)
+      : [c] "v"(c), [d] "v"(d));
+}
+
+// (a, b) *= (c, d) as fma(a, c, -(b d)), fma(a, d, b c)
+__device__ __forceinline__ void ip_cmul_fma(double& a, double& b, double c, double d) {
+  double t1, t2;
+  asm("v_mul_f64 %[t1], %[b], %[d]\n\t"
+      "v_mul_f64 %[t2], %[b], %[c]\n\t"
+      "v_fma_f64 %[b], %[a], %[d], %[t2]\n\t"
+      "v_fma_f64 %[a], %[a], %[c], -%[t1]"
+      : [a] "+v"(a), [b] "+v"(b), [t1] "=&v"(t1), [t2] "=&v"(t2)
+      : [c] "v"(c), [d] "v"(d));
+}
+
+template <bool F>
+__device__ __forceinline__ void ip_mulc(double2& x, double2 f) {
+  if (F) ip_cmul_fma(x.x, x.y, f.x, f.y);
+  else ip_cmul(x.x, x.y, f.x, f.y);
+}
+
+// (a, b) *= s (s = +-1: exact)
+__device__ __forceinline__ void ip_scale(double2& x, double s) {
+  asm("v_mul_f64 %[a], %[a], %[s]\n\t"
+      "v_mul_f64 %[b], %[b], %[s]"
+      : [a] "+v"(x.x), [b] "+v"(x.y)
+      : [s] "v"(s));
+}
+
+// x' = nf (x + y), y' = nf (x - y), nf real
+__device__ __forceinline__ void ip_h_real(double2& x, double2& y, double nf) {
+  double t1, t2, t3, t4;
+  asm("v_add_f64 %[t1], %[a], %[c]\n\t"
+      "v_add_f64 %[t2], %[b], %[d]\n\t"
+      "v_add_f64 %[t3], %[a], -%[c]\n\t"
+      "v_add_f64 %[t4], %[b], -%[d]\n\t"
+      "v_mul_f64 %[a], %[n], %[t1]\n\t"
+      "v_mul_f64 %[b], %[n], %[t2]\n\t"
+      "v_mul_f64 %[c], %[n], %[t3]\n\t"
+      "v_mul_f64 %[d], %[n], %[t4]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [t1] "=&v"(t1),
+        [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4)
+      : [n] "v"(nf));
+}
+
+// x' = nf (x + y), y' = nf (x - y), nf complex: sums first, then in-place
+// products nf * s (the product's operand order is the per-gate sweep's)
+template <bool F>
+__device__ __forceinline__ void ip_h_cplx(double2& x, double2& y, double2 nf) {
+  double2 s, d;
+  asm("v_add_f64 %[sr], %[a], %[c]\n\t"
+      "v_add_f64 %[si], %[b], %[d]\n\t"
+      "v_add_f64 %[dr], %[a], -%[c]\n\t"
+      "v_add_f64 %[di], %[b], -%[d]"
+      : [sr] "=&v"(s.x), [si] "=&v"(s.y), [dr] "=&v"(d.x), [di] "=&v"(d.y)
+      : [a] "v"(x.x), [b] "v"(x.y), [c] "v"(y.x), [d] "v"(y.y));
+  // nf * s: (nf.x s.x - nf.y s.y, nf.x s.y + nf.y s.x), written into x and y
+  double t1, t2;
+  if (F) {
+    asm("v_mul_f64 %[t1], %[ni], %[si]\n\t"
+        "v_mul_f64 %[t2], %[ni], %[sr]\n\t"
+        "v_fma_f64 %[xa], %[nr], %[sr], -%[t1]\n\t"
+        "v_fma_f64 %[xb], %[nr], %[si], %[t2]\n\t"
+        "v_mul_f64 %[t1], %[ni], %[di]\n\t"
+        "v_mul_f64 %[t2], %[ni], %[dr]\n\t"
+        "v_fma_f64 %[yc], %[nr], %[dr], -%[t1]\n\t"
+        "v_fma_f64 %[yd], %[nr], %[di], %[t2]"
+        : [xa] "+v"(x.x), [xb] "+v"(x.y), [yc] "+v"(y.x), [yd] "+v"(y.y), [t1] "=&v"(t1),
+          [t2] "=&v"(t2)
+        : [nr] "v"(nf.x), [ni] "v"(nf.y), [sr] "v"(s.x), [si] "v"(s.y), [dr] "v"(d.x),
+          [di] "v"(d.y));
+  } else {
+    asm("v_mul_f64 %[t1], %[nr], %[sr]\n\t"
+        "v_mul_f64 %[t2], %[ni], %[si]\n\t"
+        "v_add_f64 %[xa], %[t1], -%[t2]\n\t"
+        "v_mul_f64 %[t1], %[nr], %[si]\n\t"
+        "v_mul_f64 %[t2], %[ni], %[sr]\n\t"
+        "v_add_f64 %[xb], %[t1], %[t2]\n\t"
+        "v_mul_f64 %[t1], %[nr], %[dr]\n\t"
+        "v_mul_f64 %[t2], %[ni], %[di]\n\t"
+        "v_add_f64 %[yc], %[t1], -%[t2]\n\t"
+        "v_mul_f64 %[t1], %[nr], %[di]\n\t"
+        "v_mul_f64 %[t2], %[ni], %[dr]\n\t"
+        "v_add_f64 %[yd], %[t1], %[t2]"
+        : [xa] "+v"(x.x), [xb] "+v"(x.y), [yc] "+v"(y.x), [yd] "+v"(y.y), [t1] "=&v"(t1),
+          [t2] "=&v"(t2)
+        : [nr] "v"(nf.x), [ni] "v"(nf.y), [sr] "v"(s.x), [si] "v"(s.y), [dr] "v"(d.x),
+          [di] "v"(d.y));
+  }
+}
+
+// One complex product m * z into (pr, pi), num-complex order:
+// (m.x z.x - m.y z.y, m.x z.y + m.y z.x). Helper text for ip_mat2.
+#define QOA_CMUL_TO(PR, PI, MR, MI, ZR, ZI)            \
+  "v_mul_f64 %[t1], %[" MR "], %[" ZR "]\n\t"            \
+  "v_mul_f64 %[t2], %[" MI "], %[" ZI "]\n\t"            \
+  "v_mul_f64 %[t3], %[" MR "], %[" ZI "]\n\t"            \
+  "v_mul_f64 %[t4], %[" MI "], %[" ZR "]\n\t"            \
+  "v_add_f64 %[" PR "], %[t1], -%[t2]\n\t"               \
+  "v_add_f64 %[" PI "], %[t3], %[t4]\n\t"
+
+// x' = m00 x + m01 y, y' = m10 x + m11 y (cadd of two num-complex products)
+__device__ __forceinline__ void ip_mat2(double2& x, double2& y, const Coef4& k) {
+  double pr, pi, qr, qi, rr, ri, sr, si, t1, t2, t3, t4;
+  asm(QOA_CMUL_TO("pr", "pi", "m0r", "m0i", "a", "b")
+      QOA_CMUL_TO("qr", "qi", "m1r", "m1i", "c", "d")
+      QOA_CMUL_TO("rr", "ri", "m2r", "m2i", "a", "b")
+      QOA_CMUL_TO("sr", "si", "m3r", "m3i", "c", "d")
+      "v_add_f64 %[a], %[pr], %[qr]\n\t"
+      "v_add_f64 %[b], %[pi], %[qi]\n\t"
+      "v_add_f64 %[c], %[rr], %[sr]\n\t"
+      "v_add_f64 %[d], %[ri], %[si]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [pr] "=&v"(pr),
+        [pi] "=&v"(pi), [qr] "=&v"(qr), [qi] "=&v"(qi), [rr] "=&v"(rr), [ri] "=&v"(ri),
+        [sr] "=&v"(sr), [si] "=&v"(si), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3),
+        [t4] "=&v"(t4)
+      : [m0r] "v"(k.c[0].x), [m0i] "v"(k.c[0].y), [m1r] "v"(k.c[1].x), [m1i] "v"(k.c[1].y),
+        [m2r] "v"(k.c[2].x), [m2i] "v"(k.c[2].y), [m3r] "v"(k.c[3].x), [m3i] "v"(k.c[3].y));
+}
+#undef QOA_CMUL_TO
+
+// FMA form of ip_mat2 (mul2c<true>): re = fma(-m.y', z'.y, fma(m.x', z'.x,
+// fma(m.x, z.x, -(m.y z.y)))) and likewise im.
+__device__ __forceinline__ void ip_mat2_fma(double2& x, double2& y, const Coef4& k) {
+  double o0r, o0i, o1r, t;
+  asm("v_mul_f64 %[t], %[m0i], %[b]\n\t"
+      "v_fma_f64 %[o0r], %[m0r], %[a], -%[t]\n\t"
+      "v_fma_f64 %[o0r], %[m1r], %[c], %[o0r]\n\t"
+      "v_fma_f64 %[o0r], -%[m1i], %[d], %[o0r]\n\t"
+      "v_mul_f64 %[t], %[m0i], %[a]\n\t"
+      "v_fma_f64 %[o0i], %[m0r], %[b], %[t]\n\t"
+      "v_fma_f64 %[o0i], %[m1r], %[d], %[o0i]\n\t"
+      "v_fma_f64 %[o0i], %[m1i], %[c], %[o0i]\n\t"
+      "v_mul_f64 %[t], %[m2i], %[b]\n\t"
+      "v_fma_f64 %[o1r], %[m2r], %[a], -%[t]\n\t"
+      "v_fma_f64 %[o1r], %[m3r], %[c], %[o1r]\n\t"
+      "v_fma_f64 %[o1r], -%[m3i], %[d], %[o1r]\n\t"
+      "v_mul_f64 %[t], %[m2i], %[a]\n\t"
+      "v_fma_f64 %[t], %[m2r], %[b], %[t]\n\t"
+      "v_fma_f64 %[t], %[m3r], %[d], %[t]\n\t"
+      "v_fma_f64 %[d], %[m3i], %[c], %[t]\n\t"
+      "v_mov_b64 %[a], %[o0r]\n\t"
+      "v_mov_b64 %[b], %[o0i]\n\t"
+      "v_mov_b64 %[c], %[o1r]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [o0r] "=&v"(o0r),
+        [o0i] "=&v"(o0i), [o1r] "=&v"(o1r), [t] "=&v"(t)
+      : [m0r] "v"(k.c[0].x), [m0i] "v"(k.c[0].y), [m1r] "v"(k.c[1].x), [m1i] "v"(k.c[1].y),
+        [m2r] "v"(k.c[2].x), [m2i] "v"(k.c[2].y), [m3r] "v"(k.c[3].x), [m3i] "v"(k.c[3].y));
+}
+
+// Y: x' = y (0, -1), y' = x (0, 1), as the per-gate sweep's products
+__device__ __forceinline__ void ip_y(double2& x, double2& y) {
+  double t1, t2, t3, t4, u1, u2, u3, u4;
+  asm("v_mul_f64 %[t1], %[c], 0\n\t"
+      "v_mul_f64 %[t2], %[d], -1.0\n\t"
+      "v_mul_f64 %[t3], %[c], -1.0\n\t"
+      "v_mul_f64 %[t4], %[d], 0\n\t"
+      "v_mul_f64 %[u1], %[a], 0\n\t"
+      "v_mul_f64 %[u2], %[b], 1.0\n\t"
+      "v_mul_f64 %[u3], %[a], 1.0\n\t"
+      "v_mul_f64 %[u4], %[b], 0\n\t"
+      "v_add_f64 %[a], %[t1], -%[t2]\n\t"
+      "v_add_f64 %[b], %[t3], %[t4]\n\t"
+      "v_add_f64 %[c], %[u1], -%[u2]\n\t"
+      "v_add_f64 %[d], %[u3], %[u4]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [t1] "=&v"(t1),
+        [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4), [u1] "=&v"(u1), [u2] "=&v"(u2),
+        [u3] "=&v"(u3), [u4] "=&v"(u4));
+}
+
+// exchange two doubles' registers
+__device__ __forceinline__ void ip_swap(double& x, double& y) {
+  uint64_t ux = __builtin_bit_cast(uint64_t, x), uy = __builtin_bit_cast(uint64_t, y);
+  uint32_t xl = (uint32_t)ux, xh = (uint32_t)(ux >> 32);
+  uint32_t yl = (uint32_t)uy, yh = (uint32_t)(uy >> 32);
+  asm("v_swap_b32 %[xl], %[yl]\n\t"
+      "v_swap_b32 %[xh], %[yh]"
+      : [xl] "+v"(xl), [xh] "+v"(xh), [yl] "+v"(yl), [yh] "+v"(yh));
+  x = __builtin_bit_cast(double, ((uint64_t)xh << 32) | xl);
+  y = __builtin_bit_cast(double, ((uint64_t)yh << 32) | yl);
+}
+
+__device__ __forceinline__ void ip_swap2(double2& x, double2& y) {
+  ip_swap(x.x, y.x);
+  ip_swap(x.y, y.y);
+}
+
+// x = lanes of `mask` ? n : x
+__device__ __forceinline__ void ip_select(double& x, double n, uint64_t mask) {
+  uint64_t ux = __builtin_bit_cast(uint64_t, x), un = __builtin_bit_cast(uint64_t, n);
+  uint32_t xl = (uint32_t)ux, xh = (uint32_t)(ux >> 32);
+  asm("v_cndmask_b32_e64 %[xl], %[xl], %[nl], %[m]\n\t"
+      "v_cndmask_b32_e64 %[xh], %[xh], %[nh], %[m]"
+      : [xl] "+v"(xl), [xh] "+v"(xh)
+      : [nl] "v"((uint32_t)un), [nh] "v"((uint32_t)(un >> 32)), [m] "s"(mask));
+  x = __builtin_bit_cast(double, ((uint64_t)xh << 32) | xl);
+}
+
+__device__ __forceinline__ void ip_select2(double2& x, double2 n, uint64_t mask) {
+  ip_select(x.x, n.x, mask);
+  ip_select(x.y, n.y, mask);
+}
+
 template <int KIND, bool F>
 __device__ __forceinline__ void pair_math(double2 x, double2 y, const Coef4& k, double2& o0,
                                           double2& o1) {
@@ -156,6 +378,18 @@ __device__ __forceinline__ void pair_math(double2 x, double2 y, const Coef4& k, 
   }
 }
 
+// pair_math in place
+template <int KIND, bool F>
+__device__ __forceinline__ void pair_inplace(double2& x, double2& y, const Coef4& k) {
+  if (KIND == kHReal) ip_h_real(x, y, k.c[0].x);
+  else if (KIND == TK_H) ip_h_cplx<F>(x, y, k.c[0]);
+  else if (KIND == TK_M) {
+    if (F) ip_mat2_fma(x, y, k);
+    else ip_mat2(x, y, k);
+  } else if (KIND == TK_X) ip_swap2(x, y);
+  else ip_y(x, y);
+}
+
 // 2x2 op on register slot J for the registers whose slots RS are all 1.
 template <int J, int RS, int KIND, bool F>
 __device__ __forceinline__ void reg_pair(double2 (&v)[kR], const Coef4& k) {
@@ -163,11 +397,7 @@ __device__ __forceinline__ void reg_pair(double2 (&v)[kR], const Coef4& k) {
   for (int r = 0; r < kR; ++r) {
     if (r & (1 << J)) continue;
     if ((r & RS) != RS) continue;
-    const int r1 = r | (1 << J);
-    double2 o0, o1;
-    pair_math<KIND, F>(v[r], v[r1], k, o0, o1);
-    v[r] = o0;
-    v[r1] = o1;
+    pair_inplace<KIND, F>(v[r], v[r | (1 << J)], k);
   }
 }
 
@@ -190,6 +420,7 @@ __device__ __forceinline__ void reg_pair_rs(double2 (&v)[kR], const Coef4& k, ui
 template <int J, int KIND, bool F>
 __device__ __forceinline__ void reg_pair_sel(double2 (&v)[kR], const Coef4& k, uint32_t rsel,
                                              bool tok) {
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(tok);
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & (1 << J)) continue;
@@ -197,8 +428,8 @@ __device__ __forceinline__ void reg_pair_sel(double2 (&v)[kR], const Coef4& k, u
     const int r1 = r | (1 << J);
     double2 o0, o1;
     pair_math<KIND, F>(v[r], v[r1], k, o0, o1);
-    v[r] = sel2(tok, o0, v[r]);
-    v[r1] = sel2(tok, o1, v[r1]);
+    ip_select2(v[r], o0, mask);
+    ip_select2(v[r1], o1, mask);
   }
 }
 
@@ -237,7 +468,7 @@ template <int RS, bool F>
 __device__ __forceinline__ void reg_phase(double2 (&v)[kR], double2 f) {
 #pragma unroll
   for (int r = 0; r < kR; ++r)
-    if ((r & RS) == RS) v[r] = mulc<F>(v[r], f);
+    if ((r & RS) == RS) ip_mulc<F>(v[r], f);
 }
 
 template <bool F, int RS = 0>
@@ -255,7 +486,7 @@ template <int RS>
 __device__ __forceinline__ void reg_scale(double2 (&v)[kR], double s) {
 #pragma unroll
   for (int r = 0; r < kR; ++r)
-    if ((r & RS) == RS) v[r] = make_double2(v[r].x * s, v[r].y * s);
+    if ((r & RS) == RS) ip_scale(v[r], s);
 }
 
 template <int RS = 0>
@@ -270,14 +501,19 @@ __device__ __forceinline__ void reg_scale_rs(double2 (&v)[kR], double s, uint32_
 }
 
 template <int A, int B>
-__device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel, bool tok) {
+__device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel, uint32_t tl, bool tok) {
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(tok);
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & ((1 << A) | (1 << B))) continue;
     if (((uint32_t)r & rsel) != rsel) continue;
-    const double2 u = v[r | (1 << B)], w = v[r | (1 << A)];
-    v[r | (1 << B)] = sel2(tok, w, u);
-    v[r | (1 << A)] = sel2(tok, u, w);
+    if (tl == 0) {  // uniform
+      ip_swap2(v[r | (1 << A)], v[r | (1 << B)]);
+    } else {
+      const double2 u = v[r | (1 << B)], w = v[r | (1 << A)];
+      ip_select2(v[r | (1 << B)], w, mask);
+      ip_select2(v[r | (1 << A)], u, mask);
+    }
   }
 }
 
@@ -359,15 +595,8 @@ __device__ __forceinline__ void reg_flip(double2 (&v)[kR]) {
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & (1 << J)) continue;
-    const int r1 = r | (1 << J);
-    const double2 x = v[r], y = v[r1];
-    if (Y) {
-      v[r] = cmul(y, make_double2(0.0, -1.0));
-      v[r1] = cmul(x, make_double2(0.0, 1.0));
-    } else {
-      v[r] = y;
-      v[r1] = x;
-    }
+    if (Y) ip_y(v[r], v[r | (1 << J)]);
+    else ip_swap2(v[r], v[r | (1 << J)]);
   }
 }
 
@@ -377,8 +606,7 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
                                           const RegLayout& L, double2* lds, bool act) {
   if (code == 3) {
 #pragma unroll
-    for (int r = 0; r < kR; ++r)
-      if (((L.thr_l | regl(L, r)) >> b) & 1u) v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
+    for (int r = 0; r < kR; ++r) ip_scale(v[r], (((L.thr_l | regl(L, r)) >> b) & 1u) ? -1.0 : 1.0);
     return;
   }
   int slot = -1;
@@ -403,7 +631,7 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
   if (code == 2) {
     const double2 f = make_double2(0.0, ((L.thr_l >> b) & 1u) ? 1.0 : -1.0);
 #pragma unroll
-    for (int r = 0; r < kR; ++r) v[r] = cmul(v[r], f);
+    for (int r = 0; r < kR; ++r) ip_cmul(v[r].x, v[r].y, f.x, f.y);
   }
 }
 
@@ -465,23 +693,23 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const TileOp& op, const
       const uint32_t rt = (w.w26 >> 16) & 0xFFu;
       if (rt != 0xFFu) {  // target held in register slot rt: the factor is uniform per r
 #pragma unroll
-        for (int r = 0; r < kR; ++r) v[r] = mulc<F>(v[r], sel2((((uint32_t)r >> rt) & 1u) != 0, m11, m00));
+        for (int r = 0; r < kR; ++r) ip_mulc<F>(v[r], sel2((((uint32_t)r >> rt) & 1u) != 0, m11, m00));
       } else {
         const uint32_t t = (w.w0 >> 8) & 0xFFu;
         const double2 f = sel2(((L.thr_l >> t) & 1u) != 0, m11, m00);
 #pragma unroll
-        for (int r = 0; r < kR; ++r) v[r] = mulc<F>(v[r], f);
+        for (int r = 0; r < kR; ++r) ip_mulc<F>(v[r], f);
       }
     } break;
     case TK_SWAP: {
       const uint32_t a0 = s0 < s1 ? s0 : s1, b0 = s0 < s1 ? s1 : s0;
       switch (a0 * 4 + b0) {
-        case 1: reg_swap<0, 1>(v, rsel, tok); break;
-        case 2: reg_swap<0, 2>(v, rsel, tok); break;
-        case 3: reg_swap<0, 3>(v, rsel, tok); break;
-        case 6: reg_swap<1, 2>(v, rsel, tok); break;
-        case 7: reg_swap<1, 3>(v, rsel, tok); break;
-        default: reg_swap<2, 3>(v, rsel, tok); break;
+        case 1: reg_swap<0, 1>(v, rsel, tl, tok); break;
+        case 2: reg_swap<0, 2>(v, rsel, tl, tok); break;
+        case 3: reg_swap<0, 3>(v, rsel, tl, tok); break;
+        case 6: reg_swap<1, 2>(v, rsel, tl, tok); break;
+        case 7: reg_swap<1, 3>(v, rsel, tl, tok); break;
+        default: reg_swap<2, 3>(v, rsel, tl, tok); break;
       }
     } break;
     case TK_MAT4: {
@@ -587,7 +815,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         } else if (code == 3) {  // Z on an outside qubit: a tile-wide sign
           if ((((uint64_t)base ^ frame) >> q) & 1ULL) {
 #pragma unroll
-            for (int r = 0; r < kR; ++r) v[r] = make_double2(v[r].x * -1.0, v[r].y * -1.0);
+            for (int r = 0; r < kR; ++r) ip_scale(v[r], -1.0);
           }
         } else {
           frame ^= 1ULL << q;
@@ -595,7 +823,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
             const double2 f =
                 make_double2(0.0, ((((uint64_t)base ^ frame) >> q) & 1ULL) ? 1.0 : -1.0);
 #pragma unroll
-            for (int r = 0; r < kR; ++r) v[r] = cmul(v[r], f);
+            for (int r = 0; r < kR; ++r) ip_cmul(v[r].x, v[r].y, f.x, f.y);
           }
         }
       }
