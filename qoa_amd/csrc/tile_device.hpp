@@ -635,42 +635,60 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
   }
 }
 
-// The byte fields of a TileOp, read as 32-bit words (scalar loads fetch
-// whole dwords; a byte read would become a vector load and a wait):
-// w0 = kind | t << 8 | t2 << 16 | nins << 24, w1 = ins[0..3],
+// A TileOp as its 28 32-bit words, held in scalar registers: uniform loads of
+// whole dwords (a byte read would become a vector load and a wait), issued
+// one op ahead so the op loop never waits on them.
+// w0 = kind | t << 8 | t2 << 16 | nins << 24, w1 = ins[0..3], w3 = aux,
+// w4..w5 = gmask, w6..w7 = gval, w8..w23 = c[0..7], w24 = gi,
 // w25 = nq | psel << 8 | qphys[0] << 16 | qphys[1] << 24,
 // w26 = qphys[2] | rsel << 8 | rt << 16 | outside << 24, w27 = tl.
-struct OpWords {
-  uint32_t w0, w1, w25, w26, w27;
-};
-static_assert(offsetof(TileOp, ins) == 4 && offsetof(TileOp, nq) == 100 &&
+constexpr int kOpWords = 28;
+static_assert(sizeof(TileOp) == 4 * kOpWords && offsetof(TileOp, ins) == 4 &&
+                  offsetof(TileOp, aux) == 12 && offsetof(TileOp, gmask) == 16 &&
+                  offsetof(TileOp, gval) == 24 && offsetof(TileOp, c) == 32 &&
+                  offsetof(TileOp, gi) == 96 && offsetof(TileOp, nq) == 100 &&
                   offsetof(TileOp, qphys) == 102 && offsetof(TileOp, rsel) == 105 &&
                   offsetof(TileOp, rt) == 106 && offsetof(TileOp, tl) == 108,
               "TileOp word layout");
 
-__device__ __forceinline__ OpWords op_words(const TileOp& op) {
-  const uint32_t* w = reinterpret_cast<const uint32_t*>(&op);
-  return {w[0], w[1], w[25], w[26], w[27]};
+struct OpRegs {
+  uint32_t w[kOpWords];
+  __device__ __forceinline__ uint32_t kind() const { return w[0] & 0xFFu; }
+  __device__ __forceinline__ uint32_t aux() const { return w[3]; }
+  __device__ __forceinline__ uint64_t gmask() const { return w[4] | (uint64_t)w[5] << 32; }
+  __device__ __forceinline__ uint64_t gval() const { return w[6] | (uint64_t)w[7] << 32; }
+  __device__ __forceinline__ double c(int i) const {
+    return __builtin_bit_cast(double, w[8 + 2 * i] | (uint64_t)w[9 + 2 * i] << 32);
+  }
+  __device__ __forceinline__ double2 c2(int i) const { return make_double2(c(2 * i), c(2 * i + 1)); }
+  __device__ __forceinline__ uint32_t gi() const { return w[24]; }
+};
+
+__device__ __forceinline__ OpRegs load_op(const TileOp* __restrict__ p) {
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+  OpRegs r;
+#pragma unroll
+  for (int i = 0; i < kOpWords; ++i) r.w[i] = q[i];
+  return r;
 }
 
 // Applies one non-layout op whose tile condition holds. tok: this thread's
 // control bits (tl) are all set; lanes without it keep their amplitudes
 // (for diagonal ops through a factor of exactly 1).
 template <bool F>
-__device__ __forceinline__ void reg_op(double2 (&v)[kR], const TileOp& op, const OpWords& w,
-                                       bool tok, const RegLayout& L,
-                                       const double* __restrict__ pool) {
-  const uint32_t kind = w.w0 & 0xFFu;
-  const uint32_t rsel = (w.w26 >> 8) & 0xFFu;
-  const uint32_t tl = w.w27;
-  const uint32_t s0 = w.w1 & 0xFFu, s1 = (w.w1 >> 8) & 0xFFu;
+__device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool tok,
+                                       const RegLayout& L, const double* __restrict__ pool) {
+  const uint32_t kind = op.kind();
+  const uint32_t rsel = (op.w[26] >> 8) & 0xFFu;
+  const uint32_t tl = op.w[27];
+  const uint32_t s0 = op.w[1] & 0xFFu, s1 = (op.w[1] >> 8) & 0xFFu;
   switch (kind) {
     case TK_H: case TK_M: case TK_X: case TK_Y: {
       Coef4 k;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) k.c[i] = make_double2(op.c[2 * i], op.c[2 * i + 1]);
+      for (int i = 0; i < 4; ++i) k.c[i] = op.c2(i);
       if (kind == TK_H) {
-        if (op.c[1] == 0.0) reg_pair_any<kHReal, F>(v, k, s0, rsel, tl, tok);
+        if (op.c(1) == 0.0) reg_pair_any<kHReal, F>(v, k, s0, rsel, tl, tok);
         else reg_pair_any<TK_H, F>(v, k, s0, rsel, tl, tok);
       } else if (kind == TK_M) {
         reg_pair_any<TK_M, F>(v, k, s0, rsel, tl, tok);
@@ -684,18 +702,18 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const TileOp& op, const
       reg_scale_rs(v, tok ? -1.0 : 1.0, rsel);
       break;
     case TK_DPHASE: {
-      const double2 f = sel2(tok, make_double2(op.c[0], op.c[1]), make_double2(1.0, 0.0));
+      const double2 f = sel2(tok, op.c2(0), make_double2(1.0, 0.0));
       reg_phase_rs<F>(v, f, rsel);
     } break;
     case TK_RZ: {
-      const double2 m00 = make_double2(op.c[0], op.c[1]);
-      const double2 m11 = make_double2(op.c[2], op.c[3]);
-      const uint32_t rt = (w.w26 >> 16) & 0xFFu;
+      const double2 m00 = op.c2(0);
+      const double2 m11 = op.c2(1);
+      const uint32_t rt = (op.w[26] >> 16) & 0xFFu;
       if (rt != 0xFFu) {  // target held in register slot rt: the factor is uniform per r
 #pragma unroll
         for (int r = 0; r < kR; ++r) ip_mulc<F>(v[r], sel2((((uint32_t)r >> rt) & 1u) != 0, m11, m00));
       } else {
-        const uint32_t t = (w.w0 >> 8) & 0xFFu;
+        const uint32_t t = (op.w[0] >> 8) & 0xFFu;
         const double2 f = sel2(((L.thr_l >> t) & 1u) != 0, m11, m00);
 #pragma unroll
         for (int r = 0; r < kR; ++r) ip_mulc<F>(v[r], f);
@@ -714,7 +732,7 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const TileOp& op, const
     } break;
     case TK_MAT4: {
       const uint32_t a0 = s0 < s1 ? s0 : s1, b0 = s0 < s1 ? s1 : s0;
-      const double* u = pool + op.aux;
+      const double* u = pool + op.aux();
       switch (a0 * 4 + b0) {
         case 1: reg_mat4<0, 1, F>(v, u); break;
         case 2: reg_mat4<0, 2, F>(v, u); break;
@@ -786,26 +804,34 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     }
   }
 
+  OpRegs op;
+  if (nops > 0) {
+    op = load_op(ops);
+    // Consume the first op's words here: the wait for them then sits before
+    // the loop, and the loop's only wait on scalar loads is the one for the
+    // prefetched op at the end of each iteration.
+    asm volatile("" ::"s"(op.w[0]));
+  }
   for (int o = 0; o < nops; ++o) {
-    const TileOp& op = ops[o];
-    const OpWords w = op_words(op);
-    if ((w.w0 & 0xFFu) == TK_LAYOUT) {
+    const OpRegs next = load_op(ops + (o + 1 < nops ? o + 1 : o));
+    if (op.kind() == TK_LAYOUT) {
       if (act) regs_to_lds(v, L, lds);
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(w.w1 >> (8 * j));
+      for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(op.w[1] >> (8 * j));
       set_layout(L, tid, k, rpos);
       if (act) lds_to_regs(v, L, lds);
       __syncthreads();
-    } else if ((((uint64_t)base ^ frame) & op.gmask) == op.gval) {  // uniform over the tile
-      reg_op<F>(v, op, w, (L.thr_l & w.w27) == w.w27, L, pool);
+    } else if ((((uint64_t)base ^ frame) & op.gmask()) == op.gval()) {  // uniform over the tile
+      reg_op<F>(v, op, (L.thr_l & op.w[27]) == op.w[27], L, pool);
     }
-    const uint32_t nq = w.w25 & 0xFFu;
+    const uint32_t nq = op.w[25] & 0xFFu;
     if (nz.on && nq) {  // trajectory errors after the gate, operand order
-      const double p = ((w.w25 >> 8) & 0xFFu) == 1 ? nz.p1 : nz.p2;
+      const double p = ((op.w[25] >> 8) & 0xFFu) == 1 ? nz.p1 : nz.p2;
       for (uint32_t j = 0; j < nq; ++j) {
-        const uint32_t q = j == 0 ? (w.w25 >> 16) & 0xFFu : j == 1 ? w.w25 >> 24 : w.w26 & 0xFFu;
-        const uint32_t code = traj_pauli(nz.seed, nz.traj, op.gi, q, p);
+        const uint32_t q =
+            j == 0 ? (op.w[25] >> 16) & 0xFFu : j == 1 ? op.w[25] >> 24 : op.w[26] & 0xFFu;
+        const uint32_t code = traj_pauli(nz.seed, nz.traj, op.gi(), q, p);
         if (!code) continue;
         int lb = -1;
         for (int s = 0; s < k; ++s)
@@ -828,6 +854,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         }
       }
     }
+    op = next;
   }
   if (act) {
 #pragma unroll
