@@ -31,9 +31,7 @@ struct Instance {
 
 struct Round {
   WorkItem* d_items = nullptr;
-  int nitems = 0;
-  uint32_t nblocks = 0;
-  uint32_t kmax = 0;
+  std::vector<ItemGroup> groups;  // one launch each
 };
 
 }  // namespace
@@ -222,7 +220,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
   b->alg_bytes = 0.0;
   for (uint32_t r = 0; rc == QOA_OK && r < max_passes; ++r) {
     std::vector<WorkItem> items;
-    uint64_t blocks = 0;
+    std::vector<uint64_t> nblk;
     Round rd;
     for (uint32_t ii = 0; ii < b->inst.size(); ++ii) {
       const Instance& in = b->inst[ii];
@@ -241,21 +239,22 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
       w.frame_out = 2 * ii + ((r + 1) & 1);
       w.ops_begin = ops_base[in.job] + pp.op_begin;
       w.nops = pp.op_end - pp.op_begin;
-      w.block_begin = (uint32_t)blocks;
       w.k = (uint8_t)P.k;
       w.init = r == 0 ? 1 : 0;
       std::memcpy(w.spos, pp.spos, 16);
       items.push_back(w);
-      blocks += 1ULL << (in.n - P.k);
-      rd.kmax = std::max(rd.kmax, P.k);
+      nblk.push_back(1ULL << (in.n - P.k));
       b->alg_bytes += (r == 0 ? 16.0 : 32.0) * (double)(1ULL << in.n);
     }
-    if (blocks >= (1ULL << 31)) {
+    std::vector<WorkItem> grouped;
+    group_items(items, nblk, &grouped, &rd.groups);
+    items.swap(grouped);
+    bool too_many = false;
+    for (const ItemGroup& g : rd.groups) too_many = too_many || g.blocks >= (1ULL << 31);
+    if (too_many) {
       rc = set_error(QOA_ERR_ARG, "round %u has too many tiles", r);
       break;
     }
-    rd.nitems = (int)items.size();
-    rd.nblocks = (uint32_t)blocks;
     if (hipMalloc(&rd.d_items, sizeof(WorkItem) * items.size()) != hipSuccess) {
       (void)hipGetLastError();
       rc = set_error(QOA_ERR_OOM, "work items");
@@ -272,7 +271,9 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
   }
   static std::once_flag once;
   std::call_once(once, [] {
-    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t>,
+    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
+    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
   });
   *out = b;
@@ -283,11 +284,10 @@ int qoa_batch_run(qoa_batch* b) {
   if (!batch_valid(b)) return set_error(QOA_ERR_HANDLE, "invalid batch");
   QOA_HIP_CHECK(hipEventRecord(b->ev0, b->stream));
   for (const Round& rd : b->rounds) {
-    const size_t lbytes = sizeof(double2) * ((1u << rd.kmax) + (1u << rd.kmax) / 16);
-    hipLaunchKernelGGL((k_batch_reg<uint32_t>), dim3(rd.nblocks), dim3(kBatchThreads), lbytes,
-                       b->stream, b->states, rd.d_items, rd.nitems, b->d_ops, b->d_pool,
-                       b->frames);
-    QOA_HIP_CHECK(hipGetLastError());
+    for (const ItemGroup& g : rd.groups) {
+      launch_group(b->stream, g, b->states, rd.d_items, b->d_ops, b->d_pool, b->frames);
+      QOA_HIP_CHECK(hipGetLastError());
+    }
   }
   QOA_HIP_CHECK(hipEventRecord(b->ev1, b->stream));
   return QOA_OK;

@@ -9,11 +9,18 @@
 #include "qoa_internal.hpp"
 #include "tile_device.hpp"
 
+#include <algorithm>
+#include <vector>
+
 namespace qoa {
 namespace tile {
 
 constexpr uint32_t kBatchMagic = 0x51424154u;  // "QBAT"
 constexpr int kBatchThreads = 256;              // tiles of up to 12 qubits
+
+// Threads per block for a tile of 2^k amplitudes: one wave up to k = 10
+// (2^(k-4) <= 64 threads), else 2^(k-4).
+constexpr uint32_t batch_threads(uint32_t k) { return k <= 10 ? 64u : 1u << (k - kRegBits); }
 
 struct WorkItem {
   uint64_t state_off;    // first amplitude of the instance's state
@@ -29,7 +36,8 @@ struct WorkItem {
 };
 static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
 
-template <typename I>
+// NOISE = false: no item of the launch has trajectory noise.
+template <typename I, bool NOISE>
 __global__ __launch_bounds__(kBatchThreads) void k_batch_reg(double2* __restrict__ states,
                                                             const WorkItem* __restrict__ items,
                                                             int nitems,
@@ -61,10 +69,56 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch_reg(double2* __restrict
   nz.p2 = it.p2;
   nz.traj = it.traj;
   nz.frame = (it.noise && !it.init) ? frames[it.frame_in] : 0ULL;
-  const uint64_t fout = reg_tile<I, false>(states + it.state_off, base, geo, ops + it.ops_begin,
+  const uint64_t fout = reg_tile<I, false, NOISE>(states + it.state_off, base, geo, ops + it.ops_begin,
                                     (int)it.nops, pool, lds, act, it.init != 0, nz);
   // every tile of the instance computes the same frame; the first one stores it
-  if (it.noise && blockIdx.x == it.block_begin && threadIdx.x == 0) frames[it.frame_out] = fout;
+  if (NOISE && it.noise && blockIdx.x == it.block_begin && threadIdx.x == 0)
+    frames[it.frame_out] = fout;
+}
+
+// Items of one launch: all have the same block size.
+struct ItemGroup {
+  uint32_t first = 0, count = 0;  // range in the grouped item array
+  uint32_t threads = 0, kmax = kRegBits;
+  uint64_t blocks = 0;
+  bool noise = false;
+};
+
+// Orders items (with nblk[i] tiles each) into groups of equal block size,
+// stable within a group, and numbers block_begin within each group.
+inline void group_items(const std::vector<WorkItem>& in, const std::vector<uint64_t>& nblk,
+                        std::vector<WorkItem>* out, std::vector<ItemGroup>* groups) {
+  out->clear();
+  groups->clear();
+  for (uint32_t th = 64; th <= (uint32_t)kBatchThreads; th *= 2) {
+    ItemGroup g;
+    g.first = (uint32_t)out->size();
+    g.threads = th;
+    for (size_t i = 0; i < in.size(); ++i) {
+      if (batch_threads(in[i].k) != th) continue;
+      WorkItem w = in[i];
+      w.block_begin = (uint32_t)g.blocks;
+      g.blocks += nblk[i];
+      g.kmax = std::max<uint32_t>(g.kmax, w.k);
+      g.noise = g.noise || w.noise != 0;
+      out->push_back(w);
+    }
+    g.count = (uint32_t)out->size() - g.first;
+    if (g.count) groups->push_back(g);
+  }
+}
+
+// Launches one group (items already on the device at d_items + g.first).
+inline void launch_group(hipStream_t st, const ItemGroup& g, double2* states,
+                         const WorkItem* d_items, const TileOp* ops, const double* pool,
+                         uint64_t* frames) {
+  const size_t lbytes = sizeof(double2) * 2 * reg_xchg_elems(g.kmax);
+  if (g.noise)
+    hipLaunchKernelGGL((k_batch_reg<uint32_t, true>), dim3((unsigned)g.blocks), dim3(g.threads),
+                       lbytes, st, states, d_items + g.first, (int)g.count, ops, pool, frames);
+  else
+    hipLaunchKernelGGL((k_batch_reg<uint32_t, false>), dim3((unsigned)g.blocks), dim3(g.threads),
+                       lbytes, st, states, d_items + g.first, (int)g.count, ops, pool, frames);
 }
 
 }  // namespace tile

@@ -289,11 +289,12 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
 // ---- register segments inside a pass ------------------------------------------
 // Each thread of the register-resident kernel holds the kRegAmps amplitudes
 // whose local indices differ in the register bits R. Ops that mix only bits
-// of R (and every diagonal op) run in registers; moving to a new R costs one
-// LDS round trip (a TK_LAYOUT op). The same greedy list scheduling as the
+// of R (and every diagonal op) run in registers; moving to a new R costs
+// TK_LAYOUT ops (LDS exchanges in quarters of the tile). The same greedy list scheduling as the
 // pass planner picks R: ops on disjoint bits commute, so the emitted order
-// stays a valid reordering. The pass starts and ends in the layout that
-// holds the top kRegBits local bits in registers (coalesced HBM access).
+// stays a valid reordering. The pass starts in the layout that holds the top
+// kRegBits local bits in registers (coalesced HBM access) and ends in one
+// whose lowest local bits are thread bits.
 struct SegOp {
   uint32_t act, all, mixb;
 };
@@ -333,60 +334,104 @@ int seg_sim(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint3
   return cnt;
 }
 
+// Greedy choice of the next register set: starting from `seed`, add the bit
+// (or pair of bits) that lets the most ops run. The set keeps at least `keep`
+// bits of the current set `cur` (a layout change then takes one exchange).
 uint32_t seg_choose(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint32_t seed,
-                    uint32_t k, uint32_t allb) {
+                    uint32_t k, uint32_t allb, uint32_t cur, int keep) {
   uint32_t R = seed;
-  int cur = seg_sim(m, rem, R, allb);
+  int score = seg_sim(m, rem, R, allb);
+  auto room = [&](uint32_t X) {  // X can still be completed with enough bits of cur
+    const int have = __builtin_popcount(X & cur);
+    return have >= keep || kRegBits - __builtin_popcount(X) >= keep - have;
+  };
   while (__builtin_popcount(R) < kRegBits) {
     int best = -1, bests = -1;
     for (uint32_t b = 0; b < k; ++b) {
-      if (R & (1u << b)) continue;
+      if ((R & (1u << b)) || !room(R | (1u << b))) continue;
       const int sc = seg_sim(m, rem, R | (1u << b), allb);
       if (sc > bests) { bests = sc; best = (int)b; }
     }
-    if (bests <= cur && __builtin_popcount(R) + 2 <= kRegBits) {
-      int b1 = -1, b2 = -1, bp = cur;
+    if (bests <= score && __builtin_popcount(R) + 2 <= kRegBits) {
+      int b1 = -1, b2 = -1, bp = score;
       for (uint32_t b = 0; b < k; ++b) {
         if (R & (1u << b)) continue;
         for (uint32_t c = b + 1; c < k; ++c) {
-          if (R & (1u << c)) continue;
+          if ((R & (1u << c)) || !room(R | (1u << b) | (1u << c))) continue;
           const int sc = seg_sim(m, rem, R | (1u << b) | (1u << c), allb);
           if (sc > bp) { bp = sc; b1 = (int)b; b2 = (int)c; }
         }
       }
       if (b1 >= 0) {
         R |= (1u << b1) | (1u << b2);
-        cur = bp;
+        score = bp;
         continue;
       }
     }
     R |= 1u << best;
-    cur = bests;
+    score = bests;
   }
   return R;
 }
 
-TileOp layout_op(uint32_t R) {
+// Register layout of a segment: slot j holds local bit s[j]; set = the bits.
+struct RegSlots {
+  uint8_t s[kRegBits];
+  uint32_t set;
+};
+
+// TK_LAYOUT op into layout `to` from `from`: ins = the new slot bits, t < t2
+// = slots whose bit is the same in both (as many as the exchange of a 2^k
+// tile needs, reg_xchg_split(k): it runs in rounds keyed by those bits).
+TileOp layout_op(const RegSlots& from, const RegSlots& to) {
   TileOp op{};
   op.kind = TK_LAYOUT;
-  int j = 0;
-  for (int b = 0; b < 32 && j < kRegBits; ++b)
-    if (R & (1u << b)) op.ins[j++] = (uint8_t)b;
+  int kept[kRegBits], nk = 0;
+  for (int j = 0; j < kRegBits; ++j) {
+    op.ins[j] = to.s[j];
+    if (from.s[j] == to.s[j]) kept[nk++] = j;
+  }
+  op.t = (uint8_t)(nk > 0 ? kept[0] : 0);
+  op.t2 = (uint8_t)(nk > 1 ? kept[1] : op.t + 1);
   return op;
 }
 
-inline uint8_t slot_of(uint32_t R, uint32_t bitpos) {
-  return (uint8_t)__builtin_popcount(R & ((1u << bitpos) - 1u));
+// Moves layout *cur to the register set nr: bits in both keep their slots,
+// entering bits take the freed slots in ascending order, and one TK_LAYOUT op
+// changes at most kRegBits - reg_xchg_split(k) slots.
+void move_layout(RegSlots* cur, uint32_t nr, uint32_t k, std::vector<TileOp>* out) {
+  const int max_changes = kRegBits - (int)reg_xchg_split(k);
+  while (cur->set != nr) {
+    RegSlots to = *cur;
+    uint32_t in = nr & ~cur->set;
+    int changed = 0;
+    for (int j = 0; j < kRegBits && in && changed < max_changes; ++j) {
+      if (nr & (1u << to.s[j])) continue;
+      const uint32_t b = (uint32_t)__builtin_ctz(in);
+      in &= in - 1u;
+      to.set = (to.set & ~(1u << to.s[j])) | (1u << b);
+      to.s[j] = (uint8_t)b;
+      ++changed;
+    }
+    out->push_back(layout_op(*cur, to));
+    *cur = to;
+  }
 }
 
-// Splits an op's controls into register slots and thread bits for layout R.
-void reg_fields(TileOp* op, uint32_t R) {
+inline uint8_t slot_of(const RegSlots& L, uint32_t bitpos) {
+  for (int j = 0; j < kRegBits; ++j)
+    if (L.s[j] == bitpos) return (uint8_t)j;
+  return 0xFF;
+}
+
+// Splits an op's controls into register slots and thread bits for layout L.
+void reg_fields(TileOp* op, const RegSlots& L) {
   op->rsel = 0;
-  for (uint32_t b = 0; b < 32; ++b)
-    if ((op->lmask & R) & (1u << b)) op->rsel |= (uint8_t)(1u << slot_of(R, b));
-  op->tl = op->lmask & ~R;
+  for (int j = 0; j < kRegBits; ++j)
+    if (op->lmask & (1u << L.s[j])) op->rsel |= (uint8_t)(1u << j);
+  op->tl = op->lmask & ~L.set;
   op->rt = 0xFF;
-  if (op->kind == TK_RZ && (R & (1u << op->t))) op->rt = slot_of(R, op->t);
+  if (op->kind == TK_RZ && (L.set & (1u << op->t))) op->rt = slot_of(L, op->t);
 }
 
 void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& gis, uint32_t k,
@@ -397,15 +442,19 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
   for (size_t i = 0; i < ops.size(); ++i) m[i] = seg_masks(ops[i]);
   std::vector<uint32_t> rem(ops.size());
   for (uint32_t i = 0; i < rem.size(); ++i) rem[i] = i;
+  RegSlots cur;
+  for (int j = 0; j < kRegBits; ++j) cur.s[j] = (uint8_t)(k - kRegBits + j);
+  cur.set = L0;
   uint32_t R = L0;
+  const int keep = (int)reg_xchg_split(k);
   uint32_t last_gi = 0xFFFFFFFFu;
   while (!rem.empty()) {
-    const int cur = seg_sim(m, rem, R, allb);
-    const uint32_t seed = cur == 0 ? m[rem[0]].act : 0u;
-    const uint32_t nr = seg_choose(m, rem, seed, k, allb);
-    if (nr != R && (cur == 0 || seg_sim(m, rem, nr, allb) > cur)) {
+    const int ready = seg_sim(m, rem, R, allb);
+    const uint32_t seed = ready == 0 ? m[rem[0]].act : 0u;
+    const uint32_t nr = seg_choose(m, rem, seed, k, allb, R, keep);
+    if (nr != R && (ready == 0 || seg_sim(m, rem, nr, allb) > ready)) {
       R = nr;
-      out->push_back(layout_op(R));
+      move_layout(&cur, R, k, out);
     }
     BlockerT<uint32_t> blk;
     std::vector<uint32_t> next;
@@ -413,11 +462,14 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
     for (uint32_t i : rem) {
       if (blk.free(m[i].all, m[i].mixb) && (m[i].act & ~R) == 0) {
         TileOp op = ops[i];
-        if (m[i].act) {
-          op.ins[0] = slot_of(R, op.t);
-          if (op.kind == TK_SWAP || op.kind == TK_MAT4) op.ins[1] = slot_of(R, op.t2);
+        if (op.kind == TK_MAT4) {  // ins = slots of the lower and the higher local bit
+          op.ins[0] = slot_of(cur, op.t < op.t2 ? op.t : op.t2);
+          op.ins[1] = slot_of(cur, op.t < op.t2 ? op.t2 : op.t);
+        } else if (m[i].act) {
+          op.ins[0] = slot_of(cur, op.t);
+          if (op.kind == TK_SWAP) op.ins[1] = slot_of(cur, op.t2);
         }
-        reg_fields(&op, R);
+        reg_fields(&op, cur);
         out->push_back(op);
         if (gis[i] != last_gi) {
           order->push_back(gis[i]);
@@ -430,7 +482,16 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
     }
     rem.swap(next);
   }
-  if (R != L0) out->push_back(layout_op(L0));
+  // The pass stores from whatever layout it ends in; the lowest local bits
+  // (up to three) must be thread bits then, so eight consecutive lanes write
+  // 128 contiguous bytes.
+  const uint32_t low = (1u << std::min<uint32_t>(3u, k - kRegBits)) - 1u;
+  if (cur.set & low) {
+    uint32_t target = cur.set & ~low;
+    for (int b = (int)k - 1; b >= 0 && __builtin_popcount(target) < kRegBits; --b)
+      if (!(low & (1u << b))) target |= 1u << b;
+    move_layout(&cur, target, k, out);
+  }
 }
 
 }  // namespace

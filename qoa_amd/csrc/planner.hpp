@@ -18,15 +18,24 @@ enum TileKind : uint8_t {
   TK_RZ = 6,      // v *= bit t ? c[2..3] : c[0..1]
   TK_SWAP = 7,    // exchange (l|m2, l|m1) for l with both bits clear (and lmask set)
   TK_MAT4 = 8,    // 4x4 on local (t, t2) = (q1, q2); coefficients at pool[aux]
-  TK_LAYOUT = 9   // register passes: ins[0..3] = local bits held in registers
+  TK_LAYOUT = 9   // register passes: ins[0..3] = local bits of register slots 0..3;
+                  // slots t < t2 keep their bit where the exchange needs it
 };
 
 // Register-resident passes: each thread holds kRegAmps amplitudes of the tile
 // whose local indices differ in kRegBits "register bits"; the other bits
 // come from the thread index. In register mode ins[0] (and ins[1]) of a 2x2,
-// SWAP or MAT4 op hold the register slot of its target bit(s).
+// SWAP or MAT4 op hold the register slot of its target bit(s); for MAT4 the
+// slots of the lower and the higher local bit.
 constexpr int kRegBits = 4;
 constexpr int kRegAmps = 1 << kRegBits;
+// Register passes exchange amplitudes through two LDS buffers of 2^(k-q)
+// elements plus row padding, q = reg_xchg_split(k) (the exchange then takes
+// 2^q rounds): up to k = 12 a buffer holds at most 1088 elements.
+constexpr uint32_t reg_xchg_split(uint32_t k) { return k <= 10 ? 0u : k == 11 ? 1u : 2u; }
+constexpr uint32_t reg_xchg_elems(uint32_t k) {
+  return (1u << (k - reg_xchg_split(k))) + ((1u << (k - reg_xchg_split(k))) >> 4);
+}
 // PlanOptions.flags bit (internal): register mode for every tile of at least
 // kRegBits qubits (the batched kernel runs small whole-state tiles this way).
 constexpr uint32_t kPlanForceReg = 1u << 31;

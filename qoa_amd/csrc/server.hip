@@ -293,10 +293,9 @@ int launch_step(qoa_server* s) {
   // one pass of every active job that has passes left
   const uint32_t slot = s->ring_pos;
   if (s->step_used[slot]) QOA_HIP_CHECK(hipEventSynchronize(s->step_ev[slot]));
-  tile::WorkItem* items = s->h_items[slot];
+  std::vector<tile::WorkItem> built;
+  std::vector<uint64_t> nblk;
   uint32_t nitems = 0;
-  uint64_t blocks = 0;
-  uint32_t kmax = 4;
   std::vector<std::shared_ptr<SrvJob>> last;
   for (auto& j : s->active) {
     if (j->finishing || j->next_pass >= j->plan.passes.size()) continue;
@@ -306,7 +305,7 @@ int launch_step(qoa_server* s) {
     if (nitems + T > s->max_items) break;  // the rest joins the next step
     const uint64_t len = 1ULL << j->desc.n_qubits;
     for (uint32_t t = 0; t < T; ++t) {
-      tile::WorkItem& w = items[nitems++];
+      tile::WorkItem w;
       std::memset(&w, 0, sizeof(w));
       w.state_off = j->state_off / 16 + (uint64_t)t * len;
       w.seed = j->desc.noise_seed;
@@ -318,24 +317,29 @@ int launch_step(qoa_server* s) {
       w.frame_out = j->frame_block * kFrameBlock + 2 * t + ((r + 1) & 1);
       w.ops_begin = (uint32_t)j->ops_off + pp.op_begin;
       w.nops = pp.op_end - pp.op_begin;
-      w.block_begin = (uint32_t)blocks;
       w.k = (uint8_t)j->plan.k;
       w.init = r == 0 ? 1 : 0;
       std::memcpy(w.spos, pp.spos, 16);
-      blocks += len >> j->plan.k;
+      built.push_back(w);
+      nblk.push_back(len >> j->plan.k);
+      ++nitems;
     }
-    kmax = std::max(kmax, j->plan.k);
     j->next_pass++;
     if (j->next_pass == j->plan.passes.size()) last.push_back(j);
   }
   if (nitems == 0) return QOA_OK;
+  std::vector<tile::WorkItem> grouped;
+  std::vector<tile::ItemGroup> groups;
+  tile::group_items(built, nblk, &grouped, &groups);
+  tile::WorkItem* items = s->h_items[slot];
+  std::memcpy(items, grouped.data(), sizeof(tile::WorkItem) * nitems);
   QOA_HIP_CHECK(hipMemcpyAsync(s->d_items[slot], items, sizeof(tile::WorkItem) * nitems,
                                hipMemcpyHostToDevice, s->stream));
-  const size_t lbytes = sizeof(double2) * ((1u << kmax) + (1u << kmax) / 16);
-  hipLaunchKernelGGL((k_batch_reg<uint32_t>), dim3((unsigned)blocks), dim3(kBatchThreads), lbytes,
-                     s->stream, (double2*)s->arena_base, s->d_items[slot], (int)nitems, s->d_ops,
-                     (const double*)s->arena_base, s->d_frames);
-  QOA_HIP_CHECK(hipGetLastError());
+  for (const tile::ItemGroup& g : groups) {
+    tile::launch_group(s->stream, g, (double2*)s->arena_base, s->d_items[slot], s->d_ops,
+                       (const double*)s->arena_base, s->d_frames);
+    QOA_HIP_CHECK(hipGetLastError());
+  }
   for (auto& j : last) {  // sample each trajectory and copy the samples back
     const uint32_t T = j->desc.trajectories, P = (uint32_t)j->plan.passes.size();
     const uint64_t len = 1ULL << j->desc.n_qubits;
@@ -515,7 +519,9 @@ int qoa_server_create(const qoa_server_opts* opts, qoa_server** out) {
   }
   static std::once_flag once;
   std::call_once(once, [] {
-    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t>,
+    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
+    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
   });
   for (uint32_t t = 0; t < o.planner_threads; ++t) s->planners.emplace_back(planner_main, s);

@@ -517,7 +517,8 @@ __device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel, uint32
   }
 }
 
-// A < B are the register slots of the lower and higher local bit.
+// A, B are the register slots of the lower and the higher local bit (basis
+// order of the 4x4 matrix: index bit 0 = slot A).
 template <int A, int B, bool F>
 __device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u) {
 #pragma unroll
@@ -541,18 +542,140 @@ __device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u) {
   }
 }
 
-// Writes the registers to LDS under layout L (padded rows: no bank conflicts
-// for the strided layouts).
-__device__ __forceinline__ void regs_to_lds(const double2 (&v)[kR], const RegLayout& L,
-                                            double2* lds, uint32_t xl = 0) {
-#pragma unroll
-  for (int r = 0; r < kR; ++r) lds[lpad((L.thr_l | regl(L, r)) ^ xl)] = v[r];
+// ---- exchanges through LDS --------------------------------------------------
+// Two buffers of 2^(k-Q) amplitudes (padded rows: no bank conflicts for the
+// strided layouts) are used alternately: a round writes one buffer, waits at
+// a barrier and reads it back, and the next round's writes go to the other
+// buffer, so one barrier per round suffices. Q = reg_xchg_split(k) register
+// slots (A, then B), whose bits are the same before and after the exchange,
+// split it into 2^Q rounds: round c moves the amplitudes whose bits at those
+// slots read c, and a buffer index is the local index without those bits.
+// Every thread of the block calls these (act == false: barriers only).
+struct Xchg {
+  double2* lds;
+  uint32_t stride;  // elements per buffer
+  uint32_t round;
+};
+
+// local index l without bit p
+__device__ __forceinline__ uint32_t squeeze1(uint32_t l, uint32_t p) {
+  return ((l >> (p + 1)) << p) | (l & ((1u << p) - 1u));
 }
 
-__device__ __forceinline__ void lds_to_regs(double2 (&v)[kR], const RegLayout& L,
-                                            const double2* lds) {
+// l without the bits of the Q split slots (pa, then pb; Q <= 2)
+template <int Q>
+__device__ __forceinline__ uint32_t squeeze(uint32_t l, uint32_t pa, uint32_t pb) {
+  if (Q == 0) return l;
+  if (Q == 1) return squeeze1(l, pa);
+  const uint32_t hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
+  return squeeze1(squeeze1(l, hi), lo);
+}
+
+// lpad and squeeze distribute over ORs of disjoint bit sets, so a buffer
+// index is a per-thread part plus a per-register part (uniform, a sum of
+// per-slot terms; the split slots contribute nothing).
+template <int Q, int A, int B>
+__device__ __forceinline__ void slot_offsets(const RegLayout& L, uint32_t pa, uint32_t pb,
+                                             uint32_t (&o)[kRegBits]) {
 #pragma unroll
-  for (int r = 0; r < kR; ++r) v[r] = lds[lpad(L.thr_l | regl(L, r))];
+  for (int j = 0; j < kRegBits; ++j) {
+    const bool split = (Q >= 1 && j == A) || (Q == 2 && j == B);
+    o[j] = split ? 0u : lpad(squeeze<Q>(L.rb[j], pa, pb));
+  }
+}
+
+template <int Q, int A, int B>
+__device__ __forceinline__ uint32_t reg_offset(const uint32_t (&o)[kRegBits], int r) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j)
+    if (((r >> j) & 1) && !((Q >= 1 && j == A) || (Q == 2 && j == B))) x += o[j];
+  return x;
+}
+
+template <int Q, int A, int B>
+__device__ __forceinline__ constexpr int round_of(int r) {
+  return Q == 0 ? 0 : Q == 1 ? ((r >> A) & 1) : (((r >> A) & 1) | (((r >> B) & 1) << 1));
+}
+
+// Register r of every thread moves from local index tw | regl(L, r) to
+// tr | regl(N, r): tw / tr are the thread parts (register bits clear) on the
+// writing and the reading side.
+template <int Q, int A, int B>
+__device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
+                                     uint32_t tw, uint32_t tr, Xchg& X, bool act) {
+  const uint32_t pa = __builtin_ctz(L.rb[A]), pb = __builtin_ctz(L.rb[B]);
+  uint32_t ow[kRegBits], orr[kRegBits];
+  slot_offsets<Q, A, B>(L, pa, pb, ow);
+  slot_offsets<Q, A, B>(N, pa, pb, orr);
+  const uint32_t bw = lpad(squeeze<Q>(tw, pa, pb)), br = lpad(squeeze<Q>(tr, pa, pb));
+#pragma unroll
+  for (int c = 0; c < (1 << Q); ++c) {
+    double2* buf = X.lds + (X.round & 1u) * X.stride;
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < kR; ++r)
+        if (round_of<Q, A, B>(r) == c) buf[bw + reg_offset<Q, A, B>(ow, r)] = v[r];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < kR; ++r)
+        if (round_of<Q, A, B>(r) == c) v[r] = buf[br + reg_offset<Q, A, B>(orr, r)];
+    }
+    ++X.round;
+  }
+}
+
+// Exchange keyed by the first reg_xchg_split(k) of slots a < b (unchanged by
+// it); 2^reg_xchg_split(k) rounds.
+__device__ __forceinline__ void xchg_any(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
+                                         uint32_t tw, uint32_t tr, uint32_t a, uint32_t b, int k,
+                                         Xchg& X, bool act) {
+  const uint32_t q = reg_xchg_split((uint32_t)k);
+  if (q == 0) {
+    xchg<0, 0, 1>(v, L, N, tw, tr, X, act);
+  } else if (q == 1) {
+    switch (a) {  // B is not used for one split slot
+      case 0: xchg<1, 0, 1>(v, L, N, tw, tr, X, act); break;
+      case 1: xchg<1, 1, 2>(v, L, N, tw, tr, X, act); break;
+      case 2: xchg<1, 2, 3>(v, L, N, tw, tr, X, act); break;
+      default: xchg<1, 3, 0>(v, L, N, tw, tr, X, act); break;
+    }
+  } else {
+    switch (a * 4 + b) {
+      case 1: xchg<2, 0, 1>(v, L, N, tw, tr, X, act); break;
+      case 2: xchg<2, 0, 2>(v, L, N, tw, tr, X, act); break;
+      case 3: xchg<2, 0, 3>(v, L, N, tw, tr, X, act); break;
+      case 6: xchg<2, 1, 2>(v, L, N, tw, tr, X, act); break;
+      case 7: xchg<2, 1, 3>(v, L, N, tw, tr, X, act); break;
+      default: xchg<2, 2, 3>(v, L, N, tw, tr, X, act); break;
+    }
+  }
+}
+
+// Layout change L -> N; a < b = two slots holding the same bit in both.
+__device__ __forceinline__ void relayout(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
+                                         uint32_t a, uint32_t b, int k, Xchg& X, bool act) {
+  xchg_any(v, L, N, L.thr_l, N.thr_l, a, b, k, X, act);
+}
+
+template <int J, bool Y>
+__device__ __forceinline__ void reg_flip(double2 (&v)[kR]);
+
+// X flips of the local bits in xl: register bits by exchanging registers,
+// thread bits through LDS (thread t reads the amplitudes of thr_l ^ x).
+__device__ __forceinline__ void flip_bits(double2 (&v)[kR], const RegLayout& L, uint32_t xl,
+                                          int k, Xchg& X, bool act) {
+  uint32_t rmask = 0;
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) rmask |= L.rb[j];
+  if (xl & L.rb[0]) reg_flip<0, false>(v);
+  if (xl & L.rb[1]) reg_flip<1, false>(v);
+  if (xl & L.rb[2]) reg_flip<2, false>(v);
+  if (xl & L.rb[3]) reg_flip<3, false>(v);
+  const uint32_t xt = xl & ~rmask;
+  if (xt) xchg_any(v, L, L, L.thr_l, L.thr_l ^ xt, 2, 3, k, X, act);
 }
 
 // Trajectory noise state of one tile pass (on == 0: no noise).
@@ -603,7 +726,7 @@ __device__ __forceinline__ void reg_flip(double2 (&v)[kR]) {
 // Pauli `code` (1 X, 2 Y, 3 Z) on local bit b of a register-resident tile,
 // with the arithmetic of apply_{x,y,z}_simd (vectorization.rs:942-999).
 __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint32_t b,
-                                          const RegLayout& L, double2* lds, bool act) {
+                                          const RegLayout& L, int k, Xchg& X, bool act) {
   if (code == 3) {
 #pragma unroll
     for (int r = 0; r < kR; ++r) ip_scale(v[r], (((L.thr_l | regl(L, r)) >> b) & 1u) ? -1.0 : 1.0);
@@ -624,10 +747,7 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
     return;
   }
   // b is a thread bit: exchange partners through LDS
-  if (act) regs_to_lds(v, L, lds, 1u << b);
-  __syncthreads();
-  if (act) lds_to_regs(v, L, lds);
-  __syncthreads();
+  flip_bits(v, L, 1u << b, k, X, act);
   if (code == 2) {
     const double2 f = make_double2(0.0, ((L.thr_l >> b) & 1u) ? 1.0 : -1.0);
 #pragma unroll
@@ -730,16 +850,21 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool 
         default: reg_swap<2, 3>(v, rsel, tl, tok); break;
       }
     } break;
-    case TK_MAT4: {
-      const uint32_t a0 = s0 < s1 ? s0 : s1, b0 = s0 < s1 ? s1 : s0;
+    case TK_MAT4: {  // s0, s1: slots of the lower and the higher local bit
       const double* u = pool + op.aux();
-      switch (a0 * 4 + b0) {
+      switch (s0 * 4 + s1) {
         case 1: reg_mat4<0, 1, F>(v, u); break;
         case 2: reg_mat4<0, 2, F>(v, u); break;
         case 3: reg_mat4<0, 3, F>(v, u); break;
+        case 4: reg_mat4<1, 0, F>(v, u); break;
         case 6: reg_mat4<1, 2, F>(v, u); break;
         case 7: reg_mat4<1, 3, F>(v, u); break;
-        default: reg_mat4<2, 3, F>(v, u); break;
+        case 8: reg_mat4<2, 0, F>(v, u); break;
+        case 9: reg_mat4<2, 1, F>(v, u); break;
+        case 11: reg_mat4<2, 3, F>(v, u); break;
+        case 12: reg_mat4<3, 0, F>(v, u); break;
+        case 13: reg_mat4<3, 1, F>(v, u); break;
+        default: reg_mat4<3, 2, F>(v, u); break;
       }
     } break;
     default:
@@ -753,8 +878,9 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool 
 // an X or Y on a qubit outside the tile flips that bit of the X frame
 // (its phase, a tile constant, is applied at once), and the frame bits that
 // fall inside the tile are applied when the tile is loaded. Returns the frame
-// leaving the pass.
-template <typename I, bool F>
+// leaving the pass. NOISE = false compiles the noise handling out (nz.on
+// must then be 0). Waves without active threads skip the arithmetic.
+template <typename I, bool F, bool NOISE>
 __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, const Geo& geo,
                                              const TileOp* __restrict__ ops, int nops,
                                              const double* __restrict__ pool, double2* lds,
@@ -787,7 +913,10 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
 #pragma unroll
     for (int r = 0; r < kR; ++r) v[r] = a[gaddr(r)];
   }
-  uint64_t frame = nz.on ? nz.frame : 0ULL;
+  Xchg X{lds, reg_xchg_elems((uint32_t)k), 0u};
+  uint64_t frame = (NOISE && nz.on) ? nz.frame : 0ULL;
+  // threads are numbered wave by wave, so a wave is idle iff its first lane is
+  const bool wave_act = (tid & ~63u) < (1u << (k - kRegBits));
   if (frame) {
     uint64_t smask = 0;
     uint32_t xl = 0;
@@ -796,16 +925,16 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
       if ((frame >> geo.spos[j]) & 1ULL) xl |= 1u << j;
     }
     if (xl) {  // frame bits inside the tile: apply those X flips now
-      if (act) regs_to_lds(v, L, lds, xl);
-      __syncthreads();
-      if (act) lds_to_regs(v, L, lds);
-      __syncthreads();
+      flip_bits(v, L, xl, k, X, act);
       frame &= ~smask;
     }
   }
 
+  // Without noise the next op's words are loaded while the current op runs;
+  // the noise handling needs those scalar registers, so with noise each op
+  // is loaded at the top of its iteration.
   OpRegs op;
-  if (nops > 0) {
+  if (!NOISE && nops > 0) {
     op = load_op(ops);
     // Consume the first op's words here: the wait for them then sits before
     // the loop, and the loop's only wait on scalar loads is the one for the
@@ -813,20 +942,21 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     asm volatile("" ::"s"(op.w[0]));
   }
   for (int o = 0; o < nops; ++o) {
-    const OpRegs next = load_op(ops + (o + 1 < nops ? o + 1 : o));
-    if (op.kind() == TK_LAYOUT) {
-      if (act) regs_to_lds(v, L, lds);
-      __syncthreads();
+    OpRegs next;
+    if (NOISE) op = load_op(ops + o);
+    else next = load_op(ops + (o + 1 < nops ? o + 1 : o));
+    if (op.kind() == TK_LAYOUT) {  // new slot bits in ins, two unchanged slots in t, t2
 #pragma unroll
       for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(op.w[1] >> (8 * j));
-      set_layout(L, tid, k, rpos);
-      if (act) lds_to_regs(v, L, lds);
-      __syncthreads();
-    } else if ((((uint64_t)base ^ frame) & op.gmask()) == op.gval()) {  // uniform over the tile
+      RegLayout N;
+      set_layout(N, tid, k, rpos);
+      relayout(v, L, N, (op.w[0] >> 8) & 0xFFu, (op.w[0] >> 16) & 0xFFu, k, X, act);
+      L = N;
+    } else if (wave_act && (((uint64_t)base ^ frame) & op.gmask()) == op.gval()) {  // uniform
       reg_op<F>(v, op, (L.thr_l & op.w[27]) == op.w[27], L, pool);
     }
     const uint32_t nq = op.w[25] & 0xFFu;
-    if (nz.on && nq) {  // trajectory errors after the gate, operand order
+    if (NOISE && nz.on && nq) {  // trajectory errors after the gate, operand order
       const double p = ((op.w[25] >> 8) & 0xFFu) == 1 ? nz.p1 : nz.p2;
       for (uint32_t j = 0; j < nq; ++j) {
         const uint32_t q =
@@ -837,7 +967,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         for (int s = 0; s < k; ++s)
           if (geo.spos[s] == q) lb = s;
         if (lb >= 0) {
-          reg_pauli(v, code, (uint32_t)lb, L, lds, act);
+          reg_pauli(v, code, (uint32_t)lb, L, k, X, act);
         } else if (code == 3) {  // Z on an outside qubit: a tile-wide sign
           if ((((uint64_t)base ^ frame) >> q) & 1ULL) {
 #pragma unroll
@@ -854,9 +984,13 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         }
       }
     }
-    op = next;
+    if (!NOISE) op = next;
   }
-  if (act) {
+  if (act) {  // store from the final layout
+    gthr = 0;
+    for (int j = 0; j < k; ++j) gthr |= (I)((L.thr_l >> j) & 1u) << geo.spos[j];
+#pragma unroll
+    for (int j = 0; j < kRegBits; ++j) gb[j] = (I)1 << geo.spos[rpos[j]];
 #pragma unroll
     for (int r = 0; r < kR; ++r) a[gaddr(r)] = v[r];
   }

@@ -82,15 +82,22 @@ def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int) -> None:
         for j, b in enumerate(pos):
             off |= ((loc >> j) & 1) << b
         b0, b1, reg_mode = (int(x) for x in rng[p])
-        layout = list(range(k - 4, k))
         for tile in range(1 << (n - k)):
+            layout = list(range(k - 4, k))  # every tile starts in the initial layout
             base = _deposit(tile, pos)
             idx = base | off
             vr, vi = re[idx].copy(), im[idx].copy()
             for op in ops[b0:b1]:
                 kind = int(op["kind"])
                 if kind == TK_LAYOUT:
-                    layout = [int(x) for x in op["ins"]]
+                    new = [int(x) for x in op["ins"]]
+                    # the first q of slots (t, t2) keep their bit: the device
+                    # exchanges the tile in 2^q rounds keyed by those bits
+                    q = 0 if k <= 10 else 1 if k == 11 else 2
+                    for ks in [int(op["t"]), int(op["t2"])][:q]:
+                        assert layout[ks] == new[ks], "kept slots"
+                    assert sorted(new) == sorted(set(new)), "slots hold distinct bits"
+                    layout = new
                     continue
                 lm = int(op["lmask"])
                 if reg_mode:
