@@ -36,9 +36,11 @@ struct WorkItem {
 };
 static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
 
-// NOISE = false: no item of the launch has trajectory noise.
+// NOISE = false: no item of the launch has trajectory noise. At most 168
+// VGPRs: three waves per SIMD.
 template <typename I, bool NOISE>
-__global__ __launch_bounds__(kBatchThreads) void k_batch_reg(double2* __restrict__ states,
+__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(3)))
+void k_batch_reg(double2* __restrict__ states,
                                                             const WorkItem* __restrict__ items,
                                                             int nitems,
                                                             const TileOp* __restrict__ ops,
