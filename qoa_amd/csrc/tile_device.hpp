@@ -142,35 +142,57 @@ struct Coef4 {
 // per-gate sweeps, every product and sum rounded on its own; the *_fma forms
 // are the FMA arithmetic of mulc<true> / mul2c<true>.
 
+// The factor operands of these primitives are VGPRs (S = false: values that
+// may differ per lane) or SGPR pairs (S = true: wave-uniform values such as
+// an op's coefficients, read straight from the op words; every VOP3
+// instruction here reads at most one of them).
+
 // (a, b) *= (c, d): (a c - b d, a d + b c)
+#define QOA_CMUL_TEXT                \
+  "v_mul_f64 %[t1], %[a], %[c]\n\t"  \
+  "v_mul_f64 %[t2], %[b], %[d]\n\t"  \
+  "v_mul_f64 %[t3], %[a], %[d]\n\t"  \
+  "v_mul_f64 %[t4], %[b], %[c]\n\t"  \
+  "v_add_f64 %[a], %[t1], -%[t2]\n\t" \
+  "v_add_f64 %[b], %[t3], %[t4]"
+template <bool S = false>
 __device__ __forceinline__ void ip_cmul(double& a, double& b, double c, double d) {
   double t1, t2, t3, t4;
-  asm("v_mul_f64 %[t1], %[a], %[c]\n\t"
-      "v_mul_f64 %[t2], %[b], %[d]\n\t"
-      "v_mul_f64 %[t3], %[a], %[d]\n\t"
-      "v_mul_f64 %[t4], %[b], %[c]\n\t"
-      "v_add_f64 %[a], %[t1], -%[t2]\n\t"
-      "v_add_f64 %[b], %[t3], %[t4]"
-      : [a] "+v"(a), [b] "+v"(b), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3),
-        [t4] "=&v"(t4)
-      : [c] "v"(c), [d] "v"(d));
+  if constexpr (S)
+    asm(QOA_CMUL_TEXT
+        : [a] "+v"(a), [b] "+v"(b), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4)
+        : [c] "s"(c), [d] "s"(d));
+  else
+    asm(QOA_CMUL_TEXT
+        : [a] "+v"(a), [b] "+v"(b), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4)
+        : [c] "v"(c), [d] "v"(d));
 }
+#undef QOA_CMUL_TEXT
 
 // (a, b) *= (c, d) as fma(a, c, -(b d)), fma(a, d, b c)
+#define QOA_CMUL_FMA_TEXT                   \
+  "v_mul_f64 %[t1], %[b], %[d]\n\t"         \
+  "v_mul_f64 %[t2], %[b], %[c]\n\t"         \
+  "v_fma_f64 %[b], %[a], %[d], %[t2]\n\t"   \
+  "v_fma_f64 %[a], %[a], %[c], -%[t1]"
+template <bool S = false>
 __device__ __forceinline__ void ip_cmul_fma(double& a, double& b, double c, double d) {
   double t1, t2;
-  asm("v_mul_f64 %[t1], %[b], %[d]\n\t"
-      "v_mul_f64 %[t2], %[b], %[c]\n\t"
-      "v_fma_f64 %[b], %[a], %[d], %[t2]\n\t"
-      "v_fma_f64 %[a], %[a], %[c], -%[t1]"
-      : [a] "+v"(a), [b] "+v"(b), [t1] "=&v"(t1), [t2] "=&v"(t2)
-      : [c] "v"(c), [d] "v"(d));
+  if constexpr (S)
+    asm(QOA_CMUL_FMA_TEXT
+        : [a] "+v"(a), [b] "+v"(b), [t1] "=&v"(t1), [t2] "=&v"(t2)
+        : [c] "s"(c), [d] "s"(d));
+  else
+    asm(QOA_CMUL_FMA_TEXT
+        : [a] "+v"(a), [b] "+v"(b), [t1] "=&v"(t1), [t2] "=&v"(t2)
+        : [c] "v"(c), [d] "v"(d));
 }
+#undef QOA_CMUL_FMA_TEXT
 
-template <bool F>
+template <bool F, bool S = false>
 __device__ __forceinline__ void ip_mulc(double2& x, double2 f) {
-  if (F) ip_cmul_fma(x.x, x.y, f.x, f.y);
-  else ip_cmul(x.x, x.y, f.x, f.y);
+  if (F) ip_cmul_fma<S>(x.x, x.y, f.x, f.y);
+  else ip_cmul<S>(x.x, x.y, f.x, f.y);
 }
 
 // (a, b) *= s (s = +-1: exact)
@@ -181,7 +203,14 @@ __device__ __forceinline__ void ip_scale(double2& x, double s) {
       : [s] "v"(s));
 }
 
-// x' = nf (x + y), y' = nf (x - y), nf real
+// (a, b) *= -1 (exact)
+__device__ __forceinline__ void ip_neg(double2& x) {
+  asm("v_mul_f64 %[a], %[a], -1.0\n\t"
+      "v_mul_f64 %[b], %[b], -1.0"
+      : [a] "+v"(x.x), [b] "+v"(x.y));
+}
+
+// x' = nf (x + y), y' = nf (x - y), nf real and uniform
 __device__ __forceinline__ void ip_h_real(double2& x, double2& y, double nf) {
   double t1, t2, t3, t4;
   asm("v_add_f64 %[t1], %[a], %[c]\n\t"
@@ -194,11 +223,11 @@ __device__ __forceinline__ void ip_h_real(double2& x, double2& y, double nf) {
       "v_mul_f64 %[d], %[n], %[t4]"
       : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [t1] "=&v"(t1),
         [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4)
-      : [n] "v"(nf));
+      : [n] "s"(nf));
 }
 
-// x' = nf (x + y), y' = nf (x - y), nf complex: sums first, then in-place
-// products nf * s (the product's operand order is the per-gate sweep's)
+// x' = nf (x + y), y' = nf (x - y), nf complex and uniform: sums first, then
+// in-place products nf * s (the product's operand order is the per-gate sweep's)
 template <bool F>
 __device__ __forceinline__ void ip_h_cplx(double2& x, double2& y, double2 nf) {
   double2 s, d;
@@ -221,7 +250,7 @@ __device__ __forceinline__ void ip_h_cplx(double2& x, double2& y, double2 nf) {
         "v_fma_f64 %[yd], %[nr], %[di], %[t2]"
         : [xa] "+v"(x.x), [xb] "+v"(x.y), [yc] "+v"(y.x), [yd] "+v"(y.y), [t1] "=&v"(t1),
           [t2] "=&v"(t2)
-        : [nr] "v"(nf.x), [ni] "v"(nf.y), [sr] "v"(s.x), [si] "v"(s.y), [dr] "v"(d.x),
+        : [nr] "s"(nf.x), [ni] "s"(nf.y), [sr] "v"(s.x), [si] "v"(s.y), [dr] "v"(d.x),
           [di] "v"(d.y));
   } else {
     asm("v_mul_f64 %[t1], %[nr], %[sr]\n\t"
@@ -238,7 +267,7 @@ __device__ __forceinline__ void ip_h_cplx(double2& x, double2& y, double2 nf) {
         "v_add_f64 %[yd], %[t1], %[t2]"
         : [xa] "+v"(x.x), [xb] "+v"(x.y), [yc] "+v"(y.x), [yd] "+v"(y.y), [t1] "=&v"(t1),
           [t2] "=&v"(t2)
-        : [nr] "v"(nf.x), [ni] "v"(nf.y), [sr] "v"(s.x), [si] "v"(s.y), [dr] "v"(d.x),
+        : [nr] "s"(nf.x), [ni] "s"(nf.y), [sr] "v"(s.x), [si] "v"(s.y), [dr] "v"(d.x),
           [di] "v"(d.y));
   }
 }
@@ -253,7 +282,8 @@ __device__ __forceinline__ void ip_h_cplx(double2& x, double2& y, double2 nf) {
   "v_add_f64 %[" PR "], %[t1], -%[t2]\n\t"               \
   "v_add_f64 %[" PI "], %[t3], %[t4]\n\t"
 
-// x' = m00 x + m01 y, y' = m10 x + m11 y (cadd of two num-complex products)
+// x' = m00 x + m01 y, y' = m10 x + m11 y (cadd of two num-complex products;
+// uniform coefficients)
 __device__ __forceinline__ void ip_mat2(double2& x, double2& y, const Coef4& k) {
   double pr, pi, qr, qi, rr, ri, sr, si, t1, t2, t3, t4;
   asm(QOA_CMUL_TO("pr", "pi", "m0r", "m0i", "a", "b")
@@ -268,8 +298,8 @@ __device__ __forceinline__ void ip_mat2(double2& x, double2& y, const Coef4& k) 
         [pi] "=&v"(pi), [qr] "=&v"(qr), [qi] "=&v"(qi), [rr] "=&v"(rr), [ri] "=&v"(ri),
         [sr] "=&v"(sr), [si] "=&v"(si), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3),
         [t4] "=&v"(t4)
-      : [m0r] "v"(k.c[0].x), [m0i] "v"(k.c[0].y), [m1r] "v"(k.c[1].x), [m1i] "v"(k.c[1].y),
-        [m2r] "v"(k.c[2].x), [m2i] "v"(k.c[2].y), [m3r] "v"(k.c[3].x), [m3i] "v"(k.c[3].y));
+      : [m0r] "s"(k.c[0].x), [m0i] "s"(k.c[0].y), [m1r] "s"(k.c[1].x), [m1i] "s"(k.c[1].y),
+        [m2r] "s"(k.c[2].x), [m2i] "s"(k.c[2].y), [m3r] "s"(k.c[3].x), [m3i] "s"(k.c[3].y));
 }
 #undef QOA_CMUL_TO
 
@@ -298,8 +328,8 @@ __device__ __forceinline__ void ip_mat2_fma(double2& x, double2& y, const Coef4&
       "v_mov_b64 %[c], %[o1r]"
       : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [o0r] "=&v"(o0r),
         [o0i] "=&v"(o0i), [o1r] "=&v"(o1r), [t] "=&v"(t)
-      : [m0r] "v"(k.c[0].x), [m0i] "v"(k.c[0].y), [m1r] "v"(k.c[1].x), [m1i] "v"(k.c[1].y),
-        [m2r] "v"(k.c[2].x), [m2i] "v"(k.c[2].y), [m3r] "v"(k.c[3].x), [m3i] "v"(k.c[3].y));
+      : [m0r] "s"(k.c[0].x), [m0i] "s"(k.c[0].y), [m1r] "s"(k.c[1].x), [m1i] "s"(k.c[1].y),
+        [m2r] "s"(k.c[2].x), [m2i] "s"(k.c[2].y), [m3r] "s"(k.c[3].x), [m3i] "s"(k.c[3].y));
 }
 
 // Y: x' = y (0, -1), y' = x (0, 1), as the per-gate sweep's products
@@ -464,22 +494,30 @@ __device__ __forceinline__ void reg_pair_any(double2 (&v)[kR], const Coef4& k, u
 }
 
 // Diagonal factor f on the registers whose slots RS are all 1.
-template <int RS, bool F>
+template <int RS, bool F, bool S>
 __device__ __forceinline__ void reg_phase(double2 (&v)[kR], double2 f) {
 #pragma unroll
   for (int r = 0; r < kR; ++r)
-    if ((r & RS) == RS) ip_mulc<F>(v[r], f);
+    if ((r & RS) == RS) ip_mulc<F, S>(v[r], f);
 }
 
-template <bool F, int RS = 0>
+template <bool F, bool S, int RS = 0>
 __device__ __forceinline__ void reg_phase_rs(double2 (&v)[kR], double2 f, uint32_t rsel) {
   if constexpr (RS < kR) {
     if (rsel == (uint32_t)RS) {
-      reg_phase<RS, F>(v, f);
+      reg_phase<RS, F, S>(v, f);
       return;
     }
-    reg_phase_rs<F, RS + 1>(v, f, rsel);
+    reg_phase_rs<F, S, RS + 1>(v, f, rsel);
   }
+}
+
+// RZ with its target in register slot RT: the factor of register r is
+// m11 or m00 by bit RT of r (uniform).
+template <int RT, bool F>
+__device__ __forceinline__ void reg_rz(double2 (&v)[kR], double2 m00, double2 m11) {
+#pragma unroll
+  for (int r = 0; r < kR; ++r) ip_mulc<F, true>(v[r], ((r >> RT) & 1) ? m11 : m00);
 }
 
 template <int RS>
@@ -487,6 +525,24 @@ __device__ __forceinline__ void reg_scale(double2 (&v)[kR], double s) {
 #pragma unroll
   for (int r = 0; r < kR; ++r)
     if ((r & RS) == RS) ip_scale(v[r], s);
+}
+
+template <int RS>
+__device__ __forceinline__ void reg_neg(double2 (&v)[kR]) {
+#pragma unroll
+  for (int r = 0; r < kR; ++r)
+    if ((r & RS) == RS) ip_neg(v[r]);
+}
+
+template <int RS = 0>
+__device__ __forceinline__ void reg_neg_rs(double2 (&v)[kR], uint32_t rsel) {
+  if constexpr (RS < kR) {
+    if (rsel == (uint32_t)RS) {
+      reg_neg<RS>(v);
+      return;
+    }
+    reg_neg_rs<RS + 1>(v, rsel);
+  }
 }
 
 template <int RS = 0>
@@ -819,24 +875,30 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool 
       }
     } break;
     case TK_DNEG:
-      reg_scale_rs(v, tok ? -1.0 : 1.0, rsel);
+      if (tl == 0) reg_neg_rs(v, rsel);
+      else reg_scale_rs(v, tok ? -1.0 : 1.0, rsel);
       break;
-    case TK_DPHASE: {
-      const double2 f = sel2(tok, op.c2(0), make_double2(1.0, 0.0));
-      reg_phase_rs<F>(v, f, rsel);
-    } break;
+    case TK_DPHASE:
+      if (tl == 0) {  // no thread controls: one factor for the whole wave
+        reg_phase_rs<F, true>(v, op.c2(0), rsel);
+      } else {
+        reg_phase_rs<F, false>(v, sel2(tok, op.c2(0), make_double2(1.0, 0.0)), rsel);
+      }
+      break;
     case TK_RZ: {
       const double2 m00 = op.c2(0);
       const double2 m11 = op.c2(1);
-      const uint32_t rt = (op.w[26] >> 16) & 0xFFu;
-      if (rt != 0xFFu) {  // target held in register slot rt: the factor is uniform per r
+      switch ((op.w[26] >> 16) & 0xFFu) {  // register slot of the target, 0xFF: a thread bit
+        case 0: reg_rz<0, F>(v, m00, m11); break;
+        case 1: reg_rz<1, F>(v, m00, m11); break;
+        case 2: reg_rz<2, F>(v, m00, m11); break;
+        case 3: reg_rz<3, F>(v, m00, m11); break;
+        default: {
+          const uint32_t t = (op.w[0] >> 8) & 0xFFu;
+          const double2 f = sel2(((L.thr_l >> t) & 1u) != 0, m11, m00);
 #pragma unroll
-        for (int r = 0; r < kR; ++r) ip_mulc<F>(v[r], sel2((((uint32_t)r >> rt) & 1u) != 0, m11, m00));
-      } else {
-        const uint32_t t = (op.w[0] >> 8) & 0xFFu;
-        const double2 f = sel2(((L.thr_l >> t) & 1u) != 0, m11, m00);
-#pragma unroll
-        for (int r = 0; r < kR; ++r) ip_mulc<F>(v[r], f);
+          for (int r = 0; r < kR; ++r) ip_mulc<F>(v[r], f);
+        } break;
       }
     } break;
     case TK_SWAP: {
