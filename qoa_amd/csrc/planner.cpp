@@ -216,6 +216,17 @@ void set_ins(TileOp* op, uint32_t bits) {
     if (bits & (1u << j)) op->ins[op->nins++] = (uint8_t)j;
 }
 
+// Shape of a 2x2 matrix c[0..7] = m00, m01, m10, m11 (re, im) for TK_M:
+// kM2Real (every imaginary part 0), kM2Rx (m00, m11 real, m01, m10
+// imaginary), else kM2General. The register kernels skip the products by
+// those zeros: the sums they leave are the products' sums exactly, up to the
+// sign of a zero result.
+uint8_t m2_shape(const double* c) {
+  if (c[1] == 0.0 && c[3] == 0.0 && c[5] == 0.0 && c[7] == 0.0) return kM2Real;
+  if (c[1] == 0.0 && c[7] == 0.0 && c[2] == 0.0 && c[4] == 0.0) return kM2Rx;
+  return kM2General;
+}
+
 void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<TileOp>* out,
           std::vector<double>* pool, bool noise) {
   TileOp op{};
@@ -241,6 +252,7 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
     case FK_H: case FK_M: case FK_X: case FK_Y:
       op.kind = f.kind == FK_H ? TK_H : f.kind == FK_M ? TK_M : f.kind == FK_X ? TK_X : TK_Y;
       op.t = (uint8_t)loc_of(spos, k, f.t);
+      if (op.kind == TK_M) op.t2 = m2_shape(op.c);
       set_ins(&op, op.lmask | (1u << op.t));
       break;
     case FK_DNEG: case FK_DPHASE:
@@ -424,6 +436,21 @@ inline uint8_t slot_of(const RegSlots& L, uint32_t bitpos) {
   return 0xFF;
 }
 
+// Puts register bits lo and hi into slots 2 and 3 (where the kernels apply
+// 4x4 ops) by slot swaps, one TK_LAYOUT op per swap (two slots stay put).
+void place_pair(RegSlots* cur, uint32_t lo, uint32_t hi, std::vector<TileOp>* out) {
+  const uint32_t want[2] = {lo, hi};
+  for (int d = 0; d < 2; ++d) {
+    const int dst = 2 + d;
+    if (cur->s[dst] == want[d]) continue;
+    const int src = slot_of(*cur, want[d]);
+    RegSlots to = *cur;
+    std::swap(to.s[src], to.s[dst]);
+    out->push_back(layout_op(*cur, to));
+    *cur = to;
+  }
+}
+
 // Splits an op's controls into register slots and thread bits for layout L.
 void reg_fields(TileOp* op, const RegSlots& L) {
   op->rsel = 0;
@@ -462,9 +489,10 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
     for (uint32_t i : rem) {
       if (blk.free(m[i].all, m[i].mixb) && (m[i].act & ~R) == 0) {
         TileOp op = ops[i];
-        if (op.kind == TK_MAT4) {  // ins = slots of the lower and the higher local bit
-          op.ins[0] = slot_of(cur, op.t < op.t2 ? op.t : op.t2);
-          op.ins[1] = slot_of(cur, op.t < op.t2 ? op.t2 : op.t);
+        if (op.kind == TK_MAT4) {  // the lower local bit in slot 2, the higher in slot 3
+          place_pair(&cur, op.t < op.t2 ? op.t : op.t2, op.t < op.t2 ? op.t2 : op.t, out);
+          op.ins[0] = 2;
+          op.ins[1] = 3;
         } else if (m[i].act) {
           op.ins[0] = slot_of(cur, op.t);
           if (op.kind == TK_SWAP) op.ins[1] = slot_of(cur, op.t2);

@@ -10,7 +10,7 @@ namespace qoa {
 // Tile-local op kinds (device op program).
 enum TileKind : uint8_t {
   TK_H = 0,       // a' = nf*(a+b), b' = nf*(a-b)                      c[0..1] = nf
-  TK_M = 1,       // a' = m00 a + m01 b, b' = m10 a + m11 b            c[0..7]
+  TK_M = 1,       // a' = m00 a + m01 b, b' = m10 a + m11 b            c[0..7]; t2 = shape
   TK_X = 2,       // swap
   TK_Y = 3,       // a' = b*(-i), b' = a*(+i)
   TK_DNEG = 4,    // v = v.scale(-1) where (l & lmask) == lmask
@@ -22,11 +22,14 @@ enum TileKind : uint8_t {
                   // slots t < t2 keep their bit where the exchange needs it
 };
 
+// TK_M shapes (t2): general, real, or real diagonal with imaginary off-diagonal.
+constexpr uint8_t kM2General = 0, kM2Real = 1, kM2Rx = 2;
+
 // Register-resident passes: each thread holds kRegAmps amplitudes of the tile
 // whose local indices differ in kRegBits "register bits"; the other bits
 // come from the thread index. In register mode ins[0] (and ins[1]) of a 2x2,
-// SWAP or MAT4 op hold the register slot of its target bit(s); for MAT4 the
-// slots of the lower and the higher local bit.
+// SWAP or MAT4 op hold the register slot of its target bit(s); a MAT4 op's
+// lower local bit is in slot 2 and its higher one in slot 3.
 constexpr int kRegBits = 4;
 constexpr int kRegAmps = 1 << kRegBits;
 // Register passes exchange amplitudes through two LDS buffers of 2^(k-q)

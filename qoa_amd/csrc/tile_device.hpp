@@ -127,6 +127,8 @@ __device__ __forceinline__ void set_layout(RegLayout& L, uint32_t tid, int k, co
 }
 
 constexpr int kHReal = 16;  // TK_H with a real factor: a' = (nf*Re, nf*Im)
+constexpr int kMReal = 17;  // TK_M of shape kM2Real
+constexpr int kMRx = 18;    // TK_M of shape kM2Rx
 
 // Coefficients of a 2x2 op: c[0..3] = m00, m01, m10, m11 (TK_H: c[0] = nf).
 struct Coef4 {
@@ -332,6 +334,85 @@ __device__ __forceinline__ void ip_mat2_fma(double2& x, double2& y, const Coef4&
         [m2r] "s"(k.c[2].x), [m2i] "s"(k.c[2].y), [m3r] "s"(k.c[3].x), [m3i] "s"(k.c[3].y));
 }
 
+// TK_M with every imaginary part 0 (m_ij real, uniform):
+// x' = (m00 a + m01 c, m00 b + m01 d), y' = (m10 a + m11 c, m10 b + m11 d)
+// for x = (a, b), y = (c, d).
+__device__ __forceinline__ void ip_mat2_real(double2& x, double2& y, double m00, double m01,
+                                             double m10, double m11) {
+  double t1, t2, t3, t4;
+  asm("v_mul_f64 %[t1], %[p], %[a]\n\t"
+      "v_mul_f64 %[t2], %[q], %[c]\n\t"
+      "v_mul_f64 %[t3], %[r], %[a]\n\t"
+      "v_mul_f64 %[t4], %[s], %[c]\n\t"
+      "v_add_f64 %[a], %[t1], %[t2]\n\t"
+      "v_add_f64 %[c], %[t3], %[t4]\n\t"
+      "v_mul_f64 %[t1], %[p], %[b]\n\t"
+      "v_mul_f64 %[t2], %[q], %[d]\n\t"
+      "v_mul_f64 %[t3], %[r], %[b]\n\t"
+      "v_mul_f64 %[t4], %[s], %[d]\n\t"
+      "v_add_f64 %[b], %[t1], %[t2]\n\t"
+      "v_add_f64 %[d], %[t3], %[t4]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [t1] "=&v"(t1),
+        [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4)
+      : [p] "s"(m00), [q] "s"(m01), [r] "s"(m10), [s] "s"(m11));
+}
+
+// FMA form: x' = (fma(m01, c, m00 a), fma(m01, d, m00 b)), likewise y'
+__device__ __forceinline__ void ip_mat2_real_fma(double2& x, double2& y, double m00, double m01,
+                                                 double m10, double m11) {
+  double t1, t3;
+  asm("v_mul_f64 %[t1], %[p], %[a]\n\t"
+      "v_mul_f64 %[t3], %[r], %[a]\n\t"
+      "v_fma_f64 %[a], %[q], %[c], %[t1]\n\t"
+      "v_fma_f64 %[c], %[s], %[c], %[t3]\n\t"
+      "v_mul_f64 %[t1], %[p], %[b]\n\t"
+      "v_mul_f64 %[t3], %[r], %[b]\n\t"
+      "v_fma_f64 %[b], %[q], %[d], %[t1]\n\t"
+      "v_fma_f64 %[d], %[s], %[d], %[t3]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [t1] "=&v"(t1),
+        [t3] "=&v"(t3)
+      : [p] "s"(m00), [q] "s"(m01), [r] "s"(m10), [s] "s"(m11));
+}
+
+// TK_M with m00 = (p, 0), m01 = (0, q), m10 = (0, r), m11 = (s, 0):
+// x' = (p a - q d, p b + q c), y' = (s c - r b, r a + s d).
+__device__ __forceinline__ void ip_mat2_rx(double2& x, double2& y, double p, double q, double r,
+                                           double s) {
+  double t1, t2, t3, t6, t7;
+  asm("v_mul_f64 %[t1], %[p], %[a]\n\t"
+      "v_mul_f64 %[t3], %[p], %[b]\n\t"
+      "v_mul_f64 %[t7], %[r], %[a]\n\t"
+      "v_mul_f64 %[t6], %[r], %[b]\n\t"
+      "v_mul_f64 %[t2], %[q], %[d]\n\t"
+      "v_add_f64 %[a], %[t1], -%[t2]\n\t"
+      "v_mul_f64 %[t2], %[q], %[c]\n\t"
+      "v_add_f64 %[b], %[t3], %[t2]\n\t"
+      "v_mul_f64 %[t1], %[s], %[c]\n\t"
+      "v_add_f64 %[c], %[t1], -%[t6]\n\t"
+      "v_mul_f64 %[t1], %[s], %[d]\n\t"
+      "v_add_f64 %[d], %[t7], %[t1]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [t1] "=&v"(t1),
+        [t2] "=&v"(t2), [t3] "=&v"(t3), [t6] "=&v"(t6), [t7] "=&v"(t7)
+      : [p] "s"(p), [q] "s"(q), [r] "s"(r), [s] "s"(s));
+}
+
+// FMA form: x' = (fma(-q, d, p a), fma(q, c, p b)), y' = (fma(s, c, -(r b)), fma(s, d, r a))
+__device__ __forceinline__ void ip_mat2_rx_fma(double2& x, double2& y, double p, double q,
+                                               double r, double s) {
+  double t1, t3, t6, t7;
+  asm("v_mul_f64 %[t1], %[p], %[a]\n\t"
+      "v_mul_f64 %[t3], %[p], %[b]\n\t"
+      "v_mul_f64 %[t7], %[r], %[a]\n\t"
+      "v_mul_f64 %[t6], %[r], %[b]\n\t"
+      "v_fma_f64 %[a], -%[q], %[d], %[t1]\n\t"
+      "v_fma_f64 %[b], %[q], %[c], %[t3]\n\t"
+      "v_fma_f64 %[c], %[s], %[c], -%[t6]\n\t"
+      "v_fma_f64 %[d], %[s], %[d], %[t7]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [t1] "=&v"(t1),
+        [t3] "=&v"(t3), [t6] "=&v"(t6), [t7] "=&v"(t7)
+      : [p] "s"(p), [q] "s"(q), [r] "s"(r), [s] "s"(s));
+}
+
 // Y: x' = y (0, -1), y' = x (0, 1), as the per-gate sweep's products
 __device__ __forceinline__ void ip_y(double2& x, double2& y) {
   double t1, t2, t3, t4, u1, u2, u3, u4;
@@ -396,7 +477,7 @@ __device__ __forceinline__ void pair_math(double2 x, double2 y, const Coef4& k, 
   } else if (KIND == TK_H) {
     o0 = mulc<F>(k.c[0], cadd(x, y));
     o1 = mulc<F>(k.c[0], csub(x, y));
-  } else if (KIND == TK_M) {
+  } else if (KIND == TK_M || KIND == kMReal || KIND == kMRx) {
     o0 = mul2c<F>(k.c[0], x, k.c[1], y);
     o1 = mul2c<F>(k.c[2], x, k.c[3], y);
   } else if (KIND == TK_X) {
@@ -416,6 +497,12 @@ __device__ __forceinline__ void pair_inplace(double2& x, double2& y, const Coef4
   else if (KIND == TK_M) {
     if (F) ip_mat2_fma(x, y, k);
     else ip_mat2(x, y, k);
+  } else if (KIND == kMReal) {
+    if (F) ip_mat2_real_fma(x, y, k.c[0].x, k.c[1].x, k.c[2].x, k.c[3].x);
+    else ip_mat2_real(x, y, k.c[0].x, k.c[1].x, k.c[2].x, k.c[3].x);
+  } else if (KIND == kMRx) {
+    if (F) ip_mat2_rx_fma(x, y, k.c[0].x, k.c[1].y, k.c[2].y, k.c[3].x);
+    else ip_mat2_rx(x, y, k.c[0].x, k.c[1].y, k.c[2].y, k.c[3].x);
   } else if (KIND == TK_X) ip_swap2(x, y);
   else ip_y(x, y);
 }
@@ -483,12 +570,13 @@ __device__ __forceinline__ void reg_pair_any(double2 (&v)[kR], const Coef4& k, u
       case 2: reg_pair_rs<2, KIND, F>(v, k, rsel); break;
       default: reg_pair_rs<3, KIND, F>(v, k, rsel); break;
     }
-  } else {
+  } else {  // shaped 2x2 matrices take the general code here (same values)
+    constexpr int SK = (KIND == kMReal || KIND == kMRx) ? TK_M : KIND;
     switch (j) {
-      case 0: reg_pair_sel<0, KIND, F>(v, k, rsel, tok); break;
-      case 1: reg_pair_sel<1, KIND, F>(v, k, rsel, tok); break;
-      case 2: reg_pair_sel<2, KIND, F>(v, k, rsel, tok); break;
-      default: reg_pair_sel<3, KIND, F>(v, k, rsel, tok); break;
+      case 0: reg_pair_sel<0, SK, F>(v, k, rsel, tok); break;
+      case 1: reg_pair_sel<1, SK, F>(v, k, rsel, tok); break;
+      case 2: reg_pair_sel<2, SK, F>(v, k, rsel, tok); break;
+      default: reg_pair_sel<3, SK, F>(v, k, rsel, tok); break;
     }
   }
 }
@@ -867,7 +955,10 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool 
         if (op.c(1) == 0.0) reg_pair_any<kHReal, F>(v, k, s0, rsel, tl, tok);
         else reg_pair_any<TK_H, F>(v, k, s0, rsel, tl, tok);
       } else if (kind == TK_M) {
-        reg_pair_any<TK_M, F>(v, k, s0, rsel, tl, tok);
+        const uint32_t shape = (op.w[0] >> 16) & 0xFFu;  // t2
+        if (shape == kM2Real) reg_pair_any<kMReal, F>(v, k, s0, rsel, tl, tok);
+        else if (shape == kM2Rx) reg_pair_any<kMRx, F>(v, k, s0, rsel, tl, tok);
+        else reg_pair_any<TK_M, F>(v, k, s0, rsel, tl, tok);
       } else if (kind == TK_X) {
         reg_pair_any<TK_X, F>(v, k, s0, rsel, tl, tok);
       } else {
@@ -912,23 +1003,9 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool 
         default: reg_swap<2, 3>(v, rsel, tl, tok); break;
       }
     } break;
-    case TK_MAT4: {  // s0, s1: slots of the lower and the higher local bit
-      const double* u = pool + op.aux();
-      switch (s0 * 4 + s1) {
-        case 1: reg_mat4<0, 1, F>(v, u); break;
-        case 2: reg_mat4<0, 2, F>(v, u); break;
-        case 3: reg_mat4<0, 3, F>(v, u); break;
-        case 4: reg_mat4<1, 0, F>(v, u); break;
-        case 6: reg_mat4<1, 2, F>(v, u); break;
-        case 7: reg_mat4<1, 3, F>(v, u); break;
-        case 8: reg_mat4<2, 0, F>(v, u); break;
-        case 9: reg_mat4<2, 1, F>(v, u); break;
-        case 11: reg_mat4<2, 3, F>(v, u); break;
-        case 12: reg_mat4<3, 0, F>(v, u); break;
-        case 13: reg_mat4<3, 1, F>(v, u); break;
-        default: reg_mat4<3, 2, F>(v, u); break;
-      }
-    } break;
+    case TK_MAT4:  // the planner puts the lower local bit in slot 2, the higher in slot 3
+      reg_mat4<2, 3, F>(v, pool + op.aux());
+      break;
     default:
       break;
   }

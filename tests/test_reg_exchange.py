@@ -154,3 +154,26 @@ def test_planner_layout_ops_keep_split_slots(tile):
         low = (1 << min(3, k - K_REG)) - 1
         assert not any(low >> b & 1 for b in layout)
     assert seen > 0
+
+
+def test_planner_mat4_slots():
+    """A 4x4 op runs with its lower local bit in register slot 2 and its
+    higher one in slot 3."""
+    from test_planner import OP, mixed_circuit
+    g, mats = mixed_circuit(14, 300, seed=11)
+    ops, spos, rng, k = plan_program(14, g, tile_qubits=12, matrices=mats)
+    seen = 0
+    for b0, b1, reg in rng:
+        if not reg:
+            continue
+        layout = list(range(k - K_REG, k))
+        for op in ops[b0:b1]:
+            kind = int(op["kind"])
+            if kind == 9:
+                layout = [int(x) for x in op["ins"]]
+            elif kind == 8:
+                seen += 1
+                lo, hi = sorted((int(op["t"]), int(op["t2"])))
+                assert (int(op["ins"][0]), int(op["ins"][1])) == (2, 3)
+                assert layout[2] == lo and layout[3] == hi
+    assert seen > 0
