@@ -36,6 +36,8 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_TFLOPS = 157.3
 FP64_TFLOPS = 78.6
+SIMDS = 1024          # 256 CUs x 4 SIMDs; a wave64 VALU or SALU instruction issues in 4 cycles
+CLOCK_MHZ = 2400.0    # peak engine clock
 QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA = 2, 4
 
 
@@ -382,11 +384,15 @@ def roofline(dom, args) -> dict:
         out["traffic"] = prof["bytes_per_launch"]
         out["traffic_source"] = "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, " \
                                 "copy-calibrated)"
-        # the pass is bound by FP64 instruction issue, not HBM: its FP64 rate
+        # the pass is bound by instruction issue, not HBM: its FP64 rate and the
+        # share of the pass's cycles each SIMD spends issuing VALU / SALU work
         fl = prof["fp64_valu_instr"] * 64 / (k["avg_us"] * 1e-6) / 1e12
         out["fp64_valu"] = {"achieved_tflops": fl, "peak_tflops": FP64_TFLOPS,
                             "frac": fl / FP64_TFLOPS,
                             "fp64_share_of_valu_instr": prof["fp64_valu_instr"] / prof["valu_instr"]}
+        cycles = k["avg_us"] * CLOCK_MHZ
+        out["issue"] = {"valu_busy": prof["valu_instr"] / SIMDS * 4 / cycles,
+                        "salu_busy": prof["salu_instr"] / SIMDS * 4 / cycles}
     return out
 
 
