@@ -71,7 +71,7 @@ void k_batch_reg(double2* __restrict__ states,
   nz.p2 = it.p2;
   nz.traj = it.traj;
   nz.frame = (it.noise && !it.init) ? frames[it.frame_in] : 0ULL;
-  const uint64_t fout = reg_tile<I, false, NOISE>(states + it.state_off, base, geo, ops + it.ops_begin,
+  const uint64_t fout = reg_tile<I, false, NOISE, true>(states + it.state_off, base, geo, ops + it.ops_begin,
                                     (int)it.nops, pool, lds, act, it.init != 0, nz);
   // every tile of the instance computes the same frame; the first one stores it
   if (NOISE && it.noise && blockIdx.x == it.block_begin && threadIdx.x == 0)
