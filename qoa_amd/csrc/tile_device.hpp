@@ -9,6 +9,8 @@
 #include "planner.hpp"
 #include "qoa_internal.hpp"
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace qoa {
@@ -518,18 +520,34 @@ __device__ __forceinline__ void reg_pair(double2 (&v)[kR], const Coef4& k) {
   }
 }
 
-// Register controls rsel picked at compile time (a chain of uniform tests).
-template <int J, int KIND, bool F, int RS = 0>
-__device__ __forceinline__ void reg_pair_rs(double2 (&v)[kR], const Coef4& k, uint32_t rsel) {
-  if constexpr (RS < kR) {
-    if constexpr ((RS & (1 << J)) == 0) {
-      if (rsel == (uint32_t)RS) {
-        reg_pair<J, RS, KIND, F>(v, k);
-        return;
-      }
-    }
-    reg_pair_rs<J, KIND, F, RS + 1>(v, k, rsel);
+// Calls fn(std::integral_constant<int, RS>) for RS = rsel (0..15): one switch
+// (a binary search of scalar compares) instead of a chain of tests.
+template <class Fn>
+__device__ __forceinline__ void rs_switch(uint32_t rsel, Fn&& fn) {
+  switch (rsel) {
+    case 0: fn(std::integral_constant<int, 0>{}); break;
+    case 1: fn(std::integral_constant<int, 1>{}); break;
+    case 2: fn(std::integral_constant<int, 2>{}); break;
+    case 3: fn(std::integral_constant<int, 3>{}); break;
+    case 4: fn(std::integral_constant<int, 4>{}); break;
+    case 5: fn(std::integral_constant<int, 5>{}); break;
+    case 6: fn(std::integral_constant<int, 6>{}); break;
+    case 7: fn(std::integral_constant<int, 7>{}); break;
+    case 8: fn(std::integral_constant<int, 8>{}); break;
+    case 9: fn(std::integral_constant<int, 9>{}); break;
+    case 10: fn(std::integral_constant<int, 10>{}); break;
+    case 11: fn(std::integral_constant<int, 11>{}); break;
+    case 12: fn(std::integral_constant<int, 12>{}); break;
+    case 13: fn(std::integral_constant<int, 13>{}); break;
+    case 14: fn(std::integral_constant<int, 14>{}); break;
+    default: fn(std::integral_constant<int, 15>{}); break;
   }
+}
+
+// Register controls rsel picked at compile time (a switch of uniform tests).
+template <int J, int KIND, bool F>
+__device__ __forceinline__ void reg_pair_rs(double2 (&v)[kR], const Coef4& k, uint32_t rsel) {
+  rs_switch(rsel, [&](auto rs) { reg_pair<J, decltype(rs)::value, KIND, F>(v, k); });
 }
 
 // Thread-bit controls: every lane computes, lanes whose control bits are
@@ -555,7 +573,7 @@ __device__ __forceinline__ void reg_pair_sel(double2 (&v)[kR], const Coef4& k, u
 // register controls per pair.
 template <int KIND, bool F>
 __device__ __forceinline__ void reg_pair_any(double2 (&v)[kR], const Coef4& k, uint32_t j,
-                                             uint32_t rsel, uint32_t tl, bool tok) {
+                                             uint32_t rsel, uint32_t tl, uint32_t thr_l) {
   if (tl == 0 && rsel == 0) {
     switch (j) {
       case 0: reg_pair<0, 0, KIND, F>(v, k); break;
@@ -572,6 +590,7 @@ __device__ __forceinline__ void reg_pair_any(double2 (&v)[kR], const Coef4& k, u
     }
   } else {  // shaped 2x2 matrices take the general code here (same values)
     constexpr int SK = (KIND == kMReal || KIND == kMRx) ? TK_M : KIND;
+    const bool tok = (thr_l & tl) == tl;
     switch (j) {
       case 0: reg_pair_sel<0, SK, F>(v, k, rsel, tok); break;
       case 1: reg_pair_sel<1, SK, F>(v, k, rsel, tok); break;
@@ -589,15 +608,9 @@ __device__ __forceinline__ void reg_phase(double2 (&v)[kR], double2 f) {
     if ((r & RS) == RS) ip_mulc<F, S>(v[r], f);
 }
 
-template <bool F, bool S, int RS = 0>
+template <bool F, bool S>
 __device__ __forceinline__ void reg_phase_rs(double2 (&v)[kR], double2 f, uint32_t rsel) {
-  if constexpr (RS < kR) {
-    if (rsel == (uint32_t)RS) {
-      reg_phase<RS, F, S>(v, f);
-      return;
-    }
-    reg_phase_rs<F, S, RS + 1>(v, f, rsel);
-  }
+  rs_switch(rsel, [&](auto rs) { reg_phase<decltype(rs)::value, F, S>(v, f); });
 }
 
 // RZ with its target in register slot RT: the factor of register r is
@@ -622,26 +635,12 @@ __device__ __forceinline__ void reg_neg(double2 (&v)[kR]) {
     if ((r & RS) == RS) ip_neg(v[r]);
 }
 
-template <int RS = 0>
 __device__ __forceinline__ void reg_neg_rs(double2 (&v)[kR], uint32_t rsel) {
-  if constexpr (RS < kR) {
-    if (rsel == (uint32_t)RS) {
-      reg_neg<RS>(v);
-      return;
-    }
-    reg_neg_rs<RS + 1>(v, rsel);
-  }
+  rs_switch(rsel, [&](auto rs) { reg_neg<decltype(rs)::value>(v); });
 }
 
-template <int RS = 0>
-__device__ __forceinline__ void reg_scale_rs(double2 (&v)[kR], double s, uint32_t rsel) {
-  if constexpr (RS < kR) {
-    if (rsel == (uint32_t)RS) {
-      reg_scale<RS>(v, s);
-      return;
-    }
-    reg_scale_rs<RS + 1>(v, s, rsel);
-  }
+__device__ __forceinline__ void reg_scale_rs(double2 (&v)[kR], double sc, uint32_t rsel) {
+  rs_switch(rsel, [&](auto rs) { reg_scale<decltype(rs)::value>(v, sc); });
 }
 
 template <int A, int B>
@@ -938,13 +937,15 @@ __device__ __forceinline__ OpRegs load_op(const TileOp* __restrict__ p) {
 
 // Applies one non-layout op whose tile condition holds. tok: this thread's
 // control bits (tl) are all set; lanes without it keep their amplitudes
-// (for diagonal ops through a factor of exactly 1).
+// (for diagonal ops through a factor of exactly 1). It is computed only on
+// the paths with thread controls.
 template <bool F>
-__device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool tok,
-                                       const RegLayout& L, const double* __restrict__ pool) {
+__device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const RegLayout& L,
+                                       const double* __restrict__ pool) {
   const uint32_t kind = op.kind();
   const uint32_t rsel = (op.w[26] >> 8) & 0xFFu;
   const uint32_t tl = op.w[27];
+  auto tok = [&]() { return (L.thr_l & tl) == tl; };
   const uint32_t s0 = op.w[1] & 0xFFu, s1 = (op.w[1] >> 8) & 0xFFu;
   switch (kind) {
     case TK_H: case TK_M: case TK_X: case TK_Y: {
@@ -952,28 +953,28 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool 
 #pragma unroll
       for (int i = 0; i < 4; ++i) k.c[i] = op.c2(i);
       if (kind == TK_H) {
-        if (op.c(1) == 0.0) reg_pair_any<kHReal, F>(v, k, s0, rsel, tl, tok);
-        else reg_pair_any<TK_H, F>(v, k, s0, rsel, tl, tok);
+        if (op.c(1) == 0.0) reg_pair_any<kHReal, F>(v, k, s0, rsel, tl, L.thr_l);
+        else reg_pair_any<TK_H, F>(v, k, s0, rsel, tl, L.thr_l);
       } else if (kind == TK_M) {
         const uint32_t shape = (op.w[0] >> 16) & 0xFFu;  // t2
-        if (shape == kM2Real) reg_pair_any<kMReal, F>(v, k, s0, rsel, tl, tok);
-        else if (shape == kM2Rx) reg_pair_any<kMRx, F>(v, k, s0, rsel, tl, tok);
-        else reg_pair_any<TK_M, F>(v, k, s0, rsel, tl, tok);
+        if (shape == kM2Real) reg_pair_any<kMReal, F>(v, k, s0, rsel, tl, L.thr_l);
+        else if (shape == kM2Rx) reg_pair_any<kMRx, F>(v, k, s0, rsel, tl, L.thr_l);
+        else reg_pair_any<TK_M, F>(v, k, s0, rsel, tl, L.thr_l);
       } else if (kind == TK_X) {
-        reg_pair_any<TK_X, F>(v, k, s0, rsel, tl, tok);
+        reg_pair_any<TK_X, F>(v, k, s0, rsel, tl, L.thr_l);
       } else {
-        reg_pair_any<TK_Y, F>(v, k, s0, rsel, tl, tok);
+        reg_pair_any<TK_Y, F>(v, k, s0, rsel, tl, L.thr_l);
       }
     } break;
     case TK_DNEG:
       if (tl == 0) reg_neg_rs(v, rsel);
-      else reg_scale_rs(v, tok ? -1.0 : 1.0, rsel);
+      else reg_scale_rs(v, tok() ? -1.0 : 1.0, rsel);
       break;
     case TK_DPHASE:
       if (tl == 0) {  // no thread controls: one factor for the whole wave
         reg_phase_rs<F, true>(v, op.c2(0), rsel);
       } else {
-        reg_phase_rs<F, false>(v, sel2(tok, op.c2(0), make_double2(1.0, 0.0)), rsel);
+        reg_phase_rs<F, false>(v, sel2(tok(), op.c2(0), make_double2(1.0, 0.0)), rsel);
       }
       break;
     case TK_RZ: {
@@ -995,12 +996,12 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, bool 
     case TK_SWAP: {
       const uint32_t a0 = s0 < s1 ? s0 : s1, b0 = s0 < s1 ? s1 : s0;
       switch (a0 * 4 + b0) {
-        case 1: reg_swap<0, 1>(v, rsel, tl, tok); break;
-        case 2: reg_swap<0, 2>(v, rsel, tl, tok); break;
-        case 3: reg_swap<0, 3>(v, rsel, tl, tok); break;
-        case 6: reg_swap<1, 2>(v, rsel, tl, tok); break;
-        case 7: reg_swap<1, 3>(v, rsel, tl, tok); break;
-        default: reg_swap<2, 3>(v, rsel, tl, tok); break;
+        case 1: reg_swap<0, 1>(v, rsel, tl, tok()); break;
+        case 2: reg_swap<0, 2>(v, rsel, tl, tok()); break;
+        case 3: reg_swap<0, 3>(v, rsel, tl, tok()); break;
+        case 6: reg_swap<1, 2>(v, rsel, tl, tok()); break;
+        case 7: reg_swap<1, 3>(v, rsel, tl, tok()); break;
+        default: reg_swap<2, 3>(v, rsel, tl, tok()); break;
       }
     } break;
     case TK_MAT4:  // the planner puts the lower local bit in slot 2, the higher in slot 3
@@ -1052,6 +1053,8 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
 #pragma unroll
     for (int r = 0; r < kR; ++r) v[r] = a[gaddr(r)];
   }
+  // the tile's loads complete here, so the op loop carries no waits for them
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt, lgkmcnt not waited for
   Xchg X{lds, reg_xchg_elems((uint32_t)k), 0u};
   uint64_t frame = (NOISE && nz.on) ? nz.frame : 0ULL;
   // Threads are numbered wave by wave, so a wave is idle iff its first lane
@@ -1098,7 +1101,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
       relayout(v, L, N, (op.w[0] >> 8) & 0xFFu, (op.w[0] >> 16) & 0xFFu, k, X, act);
       L = N;
     } else if (wave_act && (((uint64_t)base ^ frame) & op.gmask()) == op.gval()) {  // uniform
-      reg_op<F>(v, op, (L.thr_l & op.w[27]) == op.w[27], L, pool);
+      reg_op<F>(v, op, L, pool);
     }
     const uint32_t nq = op.w[25] & 0xFFu;
     if (NOISE && nz.on && nq) {  // trajectory errors after the gate, operand order
