@@ -146,10 +146,10 @@ __global__ __launch_bounds__(kThreads) void k_fused(double2* __restrict__ a,
   for (uint32_t l = tid; l < T; l += kThreads) a[base | scatter<I>(l, geo)] = lds[l];
 }
 
-// At most 128 VGPRs: four waves per SIMD (the LDS of four 256-thread blocks
-// fits a CU).
+// At most 96 VGPRs: five waves per SIMD (five 256-thread blocks, 32 KiB of
+// LDS each, fill a CU).
 template <typename I, bool F>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_fused_reg(double2* __restrict__ a,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5))) void k_fused_reg(double2* __restrict__ a,
                                                   const TileOp* __restrict__ ops, int nops,
                                                   const double* __restrict__ pool, Geo geo) {
   extern __shared__ double2 lds[];
@@ -256,8 +256,8 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
     double2* a = (double2*)s->amps;
     timing_begin(s, KK_FUSED, 32.0 * (double)s->len);
     if (pp.reg_mode) {
-      // 16 amplitudes per thread; LDS holds two padded quarter tiles for
-      // layout changes
+      // 16 amplitudes per thread; LDS holds two quarter tiles for layout
+      // changes
       const unsigned threads = 1u << (k - kRegBits);
       const size_t lbytes = sizeof(double2) * 2 * reg_xchg_elems(k);
       const bool fma = (opt.flags & QOA_FUSE_FMA) != 0;

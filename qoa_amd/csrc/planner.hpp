@@ -33,12 +33,10 @@ constexpr uint8_t kM2General = 0, kM2Real = 1, kM2Rx = 2;
 constexpr int kRegBits = 4;
 constexpr int kRegAmps = 1 << kRegBits;
 // Register passes exchange amplitudes through two LDS buffers of 2^(k-q)
-// elements plus row padding, q = reg_xchg_split(k) (the exchange then takes
-// 2^q rounds): up to k = 12 a buffer holds at most 1088 elements.
+// elements, q = reg_xchg_split(k) (the exchange then takes 2^q rounds): up
+// to k = 12 a buffer holds at most 1024 elements (16 KiB).
 constexpr uint32_t reg_xchg_split(uint32_t k) { return k <= 10 ? 0u : k == 11 ? 1u : 2u; }
-constexpr uint32_t reg_xchg_elems(uint32_t k) {
-  return (1u << (k - reg_xchg_split(k))) + ((1u << (k - reg_xchg_split(k))) >> 4);
-}
+constexpr uint32_t reg_xchg_elems(uint32_t k) { return 1u << (k - reg_xchg_split(k)); }
 // PlanOptions.flags bit (internal): register mode for every tile of at least
 // kRegBits qubits (the batched kernel runs small whole-state tiles this way).
 constexpr uint32_t kPlanForceReg = 1u << 31;

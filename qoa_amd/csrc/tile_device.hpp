@@ -97,8 +97,6 @@ __device__ __forceinline__ double2 macc(double2 acc, double2 a, double2 x) {
 // thread bits (tl, one per-thread test per op).
 constexpr int kR = kRegAmps;
 
-__device__ __forceinline__ uint32_t lpad(uint32_t l) { return l + (l >> 4); }
-
 struct RegLayout {
   uint32_t rb[kRegBits];  // 1 << rpos[j] (uniform)
   uint32_t thr_l;         // this thread's local bits, register bits clear
@@ -686,8 +684,9 @@ __device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u) {
 }
 
 // ---- exchanges through LDS --------------------------------------------------
-// Two buffers of 2^(k-Q) amplitudes (padded rows: no bank conflicts for the
-// strided layouts) are used alternately: a round writes one buffer, waits at
+// Two buffers of 2^(k-Q) amplitudes, indexed through swz (bits 4..7 of an
+// index XORed into bits 0..3, which spreads the strided layouts over the
+// banks without padding), are used alternately: a round writes one buffer, waits at
 // a barrier and reads it back, and the next round's writes go to the other
 // buffer, so one barrier per round suffices. Q = reg_xchg_split(k) register
 // slots (A, then B), whose bits are the same before and after the exchange,
@@ -714,16 +713,20 @@ __device__ __forceinline__ uint32_t squeeze(uint32_t l, uint32_t pa, uint32_t pb
   return squeeze1(squeeze1(l, hi), lo);
 }
 
-// lpad and squeeze distribute over ORs of disjoint bit sets, so a buffer
-// index is a per-thread part plus a per-register part (uniform, a sum of
-// per-slot terms; the split slots contribute nothing).
+// 16-byte slots of a buffer: an index with bits 4..7 XORed into bits 0..3
+// (a bijection of [0, 2^m) onto itself).
+__device__ __forceinline__ uint32_t swz(uint32_t x) { return x ^ ((x >> 4) & 15u); }
+
+// swz and squeeze are linear over XOR, and disjoint bit sets OR as they XOR,
+// so a buffer index is a per-thread part XOR a per-register part (uniform,
+// the XOR of per-slot terms; the split slots contribute nothing).
 template <int Q, int A, int B>
 __device__ __forceinline__ void slot_offsets(const RegLayout& L, uint32_t pa, uint32_t pb,
                                              uint32_t (&o)[kRegBits]) {
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j) {
     const bool split = (Q >= 1 && j == A) || (Q == 2 && j == B);
-    o[j] = split ? 0u : lpad(squeeze<Q>(L.rb[j], pa, pb));
+    o[j] = split ? 0u : swz(squeeze<Q>(L.rb[j], pa, pb));
   }
 }
 
@@ -732,7 +735,7 @@ __device__ __forceinline__ uint32_t reg_offset(const uint32_t (&o)[kRegBits], in
   uint32_t x = 0;
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j)
-    if (((r >> j) & 1) && !((Q >= 1 && j == A) || (Q == 2 && j == B))) x += o[j];
+    if (((r >> j) & 1) && !((Q >= 1 && j == A) || (Q == 2 && j == B))) x ^= o[j];
   return x;
 }
 
@@ -751,20 +754,20 @@ __device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const
   uint32_t ow[kRegBits], orr[kRegBits];
   slot_offsets<Q, A, B>(L, pa, pb, ow);
   slot_offsets<Q, A, B>(N, pa, pb, orr);
-  const uint32_t bw = lpad(squeeze<Q>(tw, pa, pb)), br = lpad(squeeze<Q>(tr, pa, pb));
+  const uint32_t bw = swz(squeeze<Q>(tw, pa, pb)), br = swz(squeeze<Q>(tr, pa, pb));
 #pragma unroll
   for (int c = 0; c < (1 << Q); ++c) {
     double2* buf = X.lds + (X.round & 1u) * X.stride;
     if (act) {
 #pragma unroll
       for (int r = 0; r < kR; ++r)
-        if (round_of<Q, A, B>(r) == c) buf[bw + reg_offset<Q, A, B>(ow, r)] = v[r];
+        if (round_of<Q, A, B>(r) == c) buf[bw ^ reg_offset<Q, A, B>(ow, r)] = v[r];
     }
     __syncthreads();
     if (act) {
 #pragma unroll
       for (int r = 0; r < kR; ++r)
-        if (round_of<Q, A, B>(r) == c) v[r] = buf[br + reg_offset<Q, A, B>(orr, r)];
+        if (round_of<Q, A, B>(r) == c) v[r] = buf[br ^ reg_offset<Q, A, B>(orr, r)];
     }
     ++X.round;
   }

@@ -1,7 +1,7 @@
 """Index model of the register kernels' LDS exchanges (CPU): the 2^q rounds
 (q = 0, 1, 2 by tile size) of a layout change and of thread-bit X flips,
 with the buffer indexing written as the device code computes it (squeeze,
-lpad, per-thread part plus per-register part). Every write lands inside its buffer, no two
+the XOR swizzle, per-thread part XOR per-register part). Every write lands inside its buffer, no two
 writes of a round collide, every read finds a value written in the same
 round, and the result is the intended permutation of the tile."""
 import itertools
@@ -14,8 +14,8 @@ from tile_emulator import plan_program
 K_REG = 4
 
 
-def lpad(x):
-    return x + (x >> 4)
+def swz(x):
+    return x ^ ((x >> 4) & 15)
 
 
 def squeeze(l, bits):
@@ -29,8 +29,7 @@ def split(k):
 
 
 def stride(k):
-    q = 1 << (k - split(k))
-    return q + (q >> 4)
+    return 1 << (k - split(k))
 
 
 def set_layout(tid, k, rpos):
@@ -58,10 +57,15 @@ def xchg(vals, k, rold, rnew, tw_of, tr_of, A, B):
     assert all(rnew[j] == rold[j] for j in sl)
 
     def offs(rpos):
-        return [0 if j in sl else lpad(squeeze(1 << rpos[j], bits)) for j in range(K_REG)]
+        return [0 if j in sl else swz(squeeze(1 << rpos[j], bits)) for j in range(K_REG)]
 
     ow, orr = offs(rold), offs(rnew)
-    off = lambda o, r: sum(o[j] for j in range(K_REG) if r >> j & 1)
+    def off(o, r):
+        x = 0
+        for j in range(K_REG):
+            if r >> j & 1:
+                x ^= o[j]
+        return x
     rnd = lambda r: sum(((r >> j) & 1) << i for i, j in enumerate(sl))
     S = stride(k)
     lds = [None] * (2 * S)
@@ -70,20 +74,20 @@ def xchg(vals, k, rold, rnew, tw_of, tr_of, A, B):
         base = (c & 1) * S
         written = set()
         for t in range(T):
-            bw = lpad(squeeze(tw_of(t), bits))
+            bw = swz(squeeze(tw_of(t), bits))
             for r in range(1 << K_REG):
                 if rnd(r) != c:
                     continue
-                i = bw + off(ow, r)
+                i = bw ^ off(ow, r)
                 assert 0 <= i < S and i not in written
                 written.add(i)
                 lds[base + i] = vals[t][r]
         for t in range(T):
-            br = lpad(squeeze(tr_of(t), bits))
+            br = swz(squeeze(tr_of(t), bits))
             for r in range(1 << K_REG):
                 if rnd(r) != c:
                     continue
-                i = br + off(orr, r)
+                i = br ^ off(orr, r)
                 assert i in written
                 out[t][r] = lds[base + i]
     return out
