@@ -143,7 +143,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
           if (j >= n_jobs) break;
           const bool noisy = jobs[j].p1 > 0.0 || jobs[j].p2 > 0.0;
           PlanOptions po{std::min(tile, jobs[j].n_qubits), opts ? opts->low_qubits : 0u,
-                         (opts ? (opts->flags & QOA_CZ_REF_MASKS) : 0u) | kPlanForceReg |
+                         (opts ? (opts->flags & QOA_CZ_REF_MASKS) : 0u) | kPlanForceReg | kPlanSwaps |
                              (noisy ? kPlanNoise : 0u)};
           rcs[j] = build_plan(jobs[j].n_qubits, gates + jobs[j].gate_offset, jobs[j].n_gates,
                               mats, po, &b->plans[j]);
@@ -270,12 +270,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
     return rc;
   }
   static std::once_flag once;
-  std::call_once(once, [] {
-    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
-    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
-  });
+  std::call_once(once, batch_kernel_attrs);
   *out = b;
   return QOA_OK;
 }

@@ -158,7 +158,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int b = geo.spos[j];
     base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
   }
-  reg_tile<I, F, false, false>(a, base, geo, ops, nops, pool, lds, true, false, NoiseArgs{});
+  reg_tile<I, F, false, false, false>(a, base, geo, ops, nops, pool, lds, true, false, NoiseArgs{});
 }
 
 // FNV-1a over the bytes that determine a plan.

@@ -387,15 +387,80 @@ uint32_t seg_choose(const std::vector<SegOp>& m, const std::vector<uint32_t>& re
 }
 
 // Register layout of a segment: slot j holds local bit s[j]; set = the bits.
+// Thread index bit m stands for local bit tb[m] (nt = k - kRegBits of them).
 struct RegSlots {
   uint8_t s[kRegBits];
   uint32_t set;
+  uint8_t tb[16];
+  uint32_t nt;
 };
 
-// TK_LAYOUT op into layout `to` from `from`: ins = the new slot bits, t < t2
-// = slots whose bit is the same in both (as many as the exchange of a 2^k
-// tile needs, reg_xchg_split(k): it runs in rounds keyed by those bits).
-TileOp layout_op(const RegSlots& from, const RegSlots& to) {
+inline uint8_t slot_of(const RegSlots& L, uint32_t bitpos) {
+  for (int j = 0; j < kRegBits; ++j)
+    if (L.s[j] == bitpos) return (uint8_t)j;
+  return 0xFF;
+}
+
+inline int tpos_of(const RegSlots& L, uint32_t bitpos) {
+  for (uint32_t m = 0; m < L.nt; ++m)
+    if (L.tb[m] == bitpos) return (int)m;
+  return -1;
+}
+
+// A TK_LAYOUT op is either a whole-tile exchange (aux bit 31 clear: ins =
+// the new slot bits, t < t2 = slots that keep their bit, the thread index
+// bits then take the other local bits in ascending order) or a list of up to
+// four swaps applied in order (aux bit 31 set): bits 28..30 = count, swap i
+// in bits 7i..7i+6 = slot j (2 bits), operand (4 bits: a thread index bit m,
+// or a slot), kind (1 bit: 0 = slot j <-> thread bit m, 1 = slot j <->
+// slot); ins = the slot bits after the op.
+struct SwapList {
+  uint32_t packed = 0, n = 0;
+  void add(uint32_t j, uint32_t operand, uint32_t kind) {
+    packed |= (j | operand << 2 | kind << 6) << (7 * n);
+    ++n;
+  }
+};
+
+void emit_swaps(const RegSlots& after, SwapList* sl, std::vector<TileOp>* out) {
+  if (sl->n == 0) return;
+  TileOp op{};
+  op.kind = TK_LAYOUT;
+  for (int j = 0; j < kRegBits; ++j) op.ins[j] = after.s[j];
+  op.aux = sl->packed | sl->n << 28 | 1u << 31;
+  out->push_back(op);
+  *sl = SwapList{};
+}
+
+// Slot j <-> thread index bit m, run through slot 3: slot j takes slot 3's
+// bit, slot 3 takes thread bit m's, thread bit m takes slot j's.
+void swap_rt(RegSlots* L, uint32_t j, uint32_t m, SwapList* sl, std::vector<TileOp>* out) {
+  if (sl->n == 4) emit_swaps(*L, sl, out);
+  const uint8_t b = L->s[j];
+  L->set = (L->set & ~(1u << b)) | (1u << L->tb[m]);
+  L->s[j] = L->s[3];
+  L->s[3] = L->tb[m];
+  L->tb[m] = b;
+  sl->add(j, m, 0);
+}
+
+// Slot i <-> slot j.
+void swap_rr(RegSlots* L, uint32_t i, uint32_t j, SwapList* sl, std::vector<TileOp>* out) {
+  if (sl->n == 4) emit_swaps(*L, sl, out);
+  std::swap(L->s[i], L->s[j]);
+  sl->add(i, j, 1);
+}
+
+// Layout moves by swaps for tiles of up to 10 qubits when the kernel has
+// them (the thread index bits are then lane bits: DPP and v_permlane swaps),
+// else by whole-tile exchanges (measured faster for 12-qubit tiles: a swap
+// there costs 52-155 us at 28 qubits against 124 us for a whole exchange).
+inline bool use_swaps(uint32_t k, bool swaps) { return swaps && k <= 10; }
+
+// Whole-tile exchange op from layout `from` to `to`: ins = the new slot
+// bits, t < t2 = slots whose bit is the same in both (the exchange of a 2^k
+// tile runs in 2^reg_xchg_split(k) rounds keyed by those slots' bits).
+TileOp exchange_op(const RegSlots& from, const RegSlots& to) {
   TileOp op{};
   op.kind = TK_LAYOUT;
   int kept[kRegBits], nk = 0;
@@ -408,10 +473,10 @@ TileOp layout_op(const RegSlots& from, const RegSlots& to) {
   return op;
 }
 
-// Moves layout *cur to the register set nr: bits in both keep their slots,
-// entering bits take the freed slots in ascending order, and one TK_LAYOUT op
-// changes at most kRegBits - reg_xchg_split(k) slots.
-void move_layout(RegSlots* cur, uint32_t nr, uint32_t k, std::vector<TileOp>* out) {
+// Whole-tile exchange to the register set nr (bits in both keep their
+// slots); the thread index bits then hold the other local bits in ascending
+// order. One op changes at most kRegBits - reg_xchg_split(k) slots.
+void exchange_layout(RegSlots* cur, uint32_t nr, uint32_t k, std::vector<TileOp>* out) {
   const int max_changes = kRegBits - (int)reg_xchg_split(k);
   while (cur->set != nr) {
     RegSlots to = *cur;
@@ -425,30 +490,90 @@ void move_layout(RegSlots* cur, uint32_t nr, uint32_t k, std::vector<TileOp>* ou
       to.s[j] = (uint8_t)b;
       ++changed;
     }
-    out->push_back(layout_op(*cur, to));
+    out->push_back(exchange_op(*cur, to));
+    uint32_t m = 0;
+    for (uint32_t b = 0; b < k; ++b)
+      if (!(to.set & (1u << b))) to.tb[m++] = (uint8_t)b;
     *cur = to;
   }
 }
 
-inline uint8_t slot_of(const RegSlots& L, uint32_t bitpos) {
-  for (int j = 0; j < kRegBits; ++j)
-    if (L.s[j] == bitpos) return (uint8_t)j;
-  return 0xFF;
+// Moves layout *cur to the register set nr, by swaps (bits in both keep
+// their slots; each entering bit trades places with a leaving one) or by
+// whole-tile exchanges, whichever costs less.
+void move_layout(RegSlots* cur, uint32_t nr, uint32_t k, bool swaps, std::vector<TileOp>* out) {
+  if (cur->set == nr) return;
+  if (!use_swaps(k, swaps)) {
+    exchange_layout(cur, nr, k, out);
+    return;
+  }
+  SwapList sl;
+  uint32_t in = nr & ~cur->set;
+  while (in) {  // a slot whose bit leaves takes an entering bit (through slot 3)
+    int j = 0;
+    while (nr & (1u << cur->s[j])) ++j;
+    const uint32_t b = (uint32_t)__builtin_ctz(in);
+    in &= in - 1u;
+    swap_rt(cur, (uint32_t)j, (uint32_t)tpos_of(*cur, b), &sl, out);
+  }
+  emit_swaps(*cur, &sl, out);
 }
 
 // Puts register bits lo and hi into slots 2 and 3 (where the kernels apply
-// 4x4 ops) by slot swaps, one TK_LAYOUT op per swap (two slots stay put).
-void place_pair(RegSlots* cur, uint32_t lo, uint32_t hi, std::vector<TileOp>* out) {
+// 4x4 ops) by slot swaps: one swap list when the kernel takes swap lists,
+// else one whole-tile exchange per swap (two slots stay put; the thread
+// bits do not change).
+void place_pair(RegSlots* cur, uint32_t lo, uint32_t hi, bool swaps, std::vector<TileOp>* out) {
+  SwapList sl;
   const uint32_t want[2] = {lo, hi};
   for (int d = 0; d < 2; ++d) {
     const int dst = 2 + d;
     if (cur->s[dst] == want[d]) continue;
-    const int src = slot_of(*cur, want[d]);
-    RegSlots to = *cur;
-    std::swap(to.s[src], to.s[dst]);
-    out->push_back(layout_op(*cur, to));
-    *cur = to;
+    const uint32_t src = slot_of(*cur, want[d]);
+    if (swaps) {
+      swap_rr(cur, src, (uint32_t)dst, &sl, out);
+    } else {
+      RegSlots to = *cur;
+      std::swap(to.s[src], to.s[dst]);
+      out->push_back(exchange_op(*cur, to));
+      *cur = to;
+    }
   }
+  emit_swaps(*cur, &sl, out);
+}
+
+// The pass stores from its final layout: the lowest local bits (up to three)
+// must then sit on thread index bits below three (any order), so eight
+// consecutive lanes write 128 contiguous bytes.
+void store_layout(RegSlots* cur, uint32_t k, bool swaps, std::vector<TileOp>* out) {
+  const uint32_t nlow = std::min<uint32_t>(3u, k - kRegBits);
+  if (!use_swaps(k, swaps)) {  // exchanges leave the thread bits in ascending order:
+    const uint32_t low = (1u << nlow) - 1u;  // keep the low bits out of the registers
+    if (cur->set & low) {
+      uint32_t target = cur->set & ~low;
+      for (int b = (int)k - 1; b >= 0 && __builtin_popcount(target) < kRegBits; --b)
+        if (!(low & (1u << b))) target |= 1u << b;
+      exchange_layout(cur, target, k, out);
+    }
+    return;
+  }
+  SwapList sl;
+  for (uint32_t b = 0; b < nlow; ++b) {
+    const int m = tpos_of(*cur, b);
+    if (m >= 0 && (uint32_t)m < nlow) continue;
+    int p = -1;  // a low thread index bit holding a bit that is not low
+    for (uint32_t q = 0; q < nlow && p < 0; ++q)
+      if (cur->tb[q] >= nlow) p = (int)q;
+    uint32_t j = slot_of(*cur, b);
+    if (j == 0xFF) {  // b sits on a higher thread index bit: bring it in first
+      j = 0;
+      while (cur->s[j] < nlow) ++j;
+      swap_rt(cur, j, (uint32_t)m, &sl, out);
+      j = slot_of(*cur, b);
+    }
+    swap_rt(cur, j, (uint32_t)p, &sl, out);
+  }
+  emit_swaps(*cur, &sl, out);
 }
 
 // Splits an op's controls into register slots and thread bits for layout L.
@@ -462,7 +587,7 @@ void reg_fields(TileOp* op, const RegSlots& L) {
 }
 
 void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& gis, uint32_t k,
-                  std::vector<TileOp>* out, std::vector<uint32_t>* order) {
+                  bool swaps, std::vector<TileOp>* out, std::vector<uint32_t>* order) {
   const uint32_t L0 = ((1u << kRegBits) - 1u) << (k - kRegBits);
   const uint32_t allb = (1u << k) - 1u;
   std::vector<SegOp> m(ops.size());
@@ -472,8 +597,10 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
   RegSlots cur;
   for (int j = 0; j < kRegBits; ++j) cur.s[j] = (uint8_t)(k - kRegBits + j);
   cur.set = L0;
+  cur.nt = k - kRegBits;
+  for (uint32_t m = 0; m < cur.nt; ++m) cur.tb[m] = (uint8_t)m;
   uint32_t R = L0;
-  const int keep = (int)reg_xchg_split(k);
+  const int keep = use_swaps(k, swaps) ? 0 : (int)reg_xchg_split(k);  // slots an exchange keeps
   uint32_t last_gi = 0xFFFFFFFFu;
   while (!rem.empty()) {
     const int ready = seg_sim(m, rem, R, allb);
@@ -481,7 +608,7 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
     const uint32_t nr = seg_choose(m, rem, seed, k, allb, R, keep);
     if (nr != R && (ready == 0 || seg_sim(m, rem, nr, allb) > ready)) {
       R = nr;
-      move_layout(&cur, R, k, out);
+      move_layout(&cur, R, k, swaps, out);
     }
     BlockerT<uint32_t> blk;
     std::vector<uint32_t> next;
@@ -490,7 +617,8 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
       if (blk.free(m[i].all, m[i].mixb) && (m[i].act & ~R) == 0) {
         TileOp op = ops[i];
         if (op.kind == TK_MAT4) {  // the lower local bit in slot 2, the higher in slot 3
-          place_pair(&cur, op.t < op.t2 ? op.t : op.t2, op.t < op.t2 ? op.t2 : op.t, out);
+          place_pair(&cur, op.t < op.t2 ? op.t : op.t2, op.t < op.t2 ? op.t2 : op.t,
+                     use_swaps(k, swaps), out);
           op.ins[0] = 2;
           op.ins[1] = 3;
         } else if (m[i].act) {
@@ -510,16 +638,7 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
     }
     rem.swap(next);
   }
-  // The pass stores from whatever layout it ends in; the lowest local bits
-  // (up to three) must be thread bits then, so eight consecutive lanes write
-  // 128 contiguous bytes.
-  const uint32_t low = (1u << std::min<uint32_t>(3u, k - kRegBits)) - 1u;
-  if (cur.set & low) {
-    uint32_t target = cur.set & ~low;
-    for (int b = (int)k - 1; b >= 0 && __builtin_popcount(target) < kRegBits; --b)
-      if (!(low & (1u << b))) target |= 1u << b;
-    move_layout(&cur, target, k, out);
-  }
+  store_layout(&cur, k, swaps, out);
 }
 
 }  // namespace
@@ -628,7 +747,7 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
       }
     }
     if (pp.reg_mode) {
-      segment_pass(pass_ops, pass_gi, k, &plan->ops, &plan->order);
+      segment_pass(pass_ops, pass_gi, k, (opt.flags & kPlanSwaps) != 0, &plan->ops, &plan->order);
     } else {
       plan->ops.insert(plan->ops.end(), pass_ops.begin(), pass_ops.end());
       for (size_t e = 0; e < pass_gi.size(); ++e)

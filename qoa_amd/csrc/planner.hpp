@@ -42,6 +42,9 @@ constexpr uint32_t reg_xchg_elems(uint32_t k) { return 1u << (k - reg_xchg_split
 constexpr uint32_t kPlanForceReg = 1u << 31;
 // PlanOptions.flags bit (internal): attach trajectory-noise operands to ops.
 constexpr uint32_t kPlanNoise = 1u << 30;
+// PlanOptions.flags bit (internal): tiles of up to 10 qubits change their
+// register layout by swaps (the batched kernel); else by whole-tile exchanges.
+constexpr uint32_t kPlanSwaps = 1u << 29;
 
 // One op of a pass program. `ins` holds the sorted local bit positions into
 // which a zero is deposited to enumerate the visited indices (nins == 0xFF:

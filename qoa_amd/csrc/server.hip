@@ -189,7 +189,7 @@ void planner_main(qoa_server* s) {
     }
     const bool noisy = j->desc.p1 > 0.0 || j->desc.p2 > 0.0;
     PlanOptions po{std::min(s->opts.tile_qubits, j->desc.n_qubits), 0u,
-                   (s->opts.flags & QOA_CZ_REF_MASKS) | kPlanForceReg | (noisy ? kPlanNoise : 0u)};
+                   (s->opts.flags & QOA_CZ_REF_MASKS) | kPlanForceReg | kPlanSwaps | (noisy ? kPlanNoise : 0u)};
     const int rc = build_plan(j->desc.n_qubits, j->gates.data(), (uint32_t)j->gates.size(),
                               j->mats.empty() ? nullptr : j->mats.data(), po, &j->plan);
     {
@@ -518,12 +518,7 @@ int qoa_server_create(const qoa_server_opts* opts, qoa_server** out) {
     s->free_pinned.push_back(o.max_jobs - 1 - b);
   }
   static std::once_flag once;
-  std::call_once(once, [] {
-    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
-    (void)hipFuncSetAttribute((const void*)k_batch_reg<uint32_t, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
-  });
+  std::call_once(once, tile::batch_kernel_attrs);
   for (uint32_t t = 0; t < o.planner_threads; ++t) s->planners.emplace_back(planner_main, s);
   s->engine = std::thread(engine_main, s);
   *out = s;

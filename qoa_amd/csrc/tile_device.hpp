@@ -800,10 +800,197 @@ __device__ __forceinline__ void xchg_any(double2 (&v)[kR], const RegLayout& L, c
   }
 }
 
-// Layout change L -> N; a < b = two slots holding the same bit in both.
+// Whole-tile layout change L -> N (N's thread bits in ascending order); a < b
+// = slots holding the same bit in both.
 __device__ __forceinline__ void relayout(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
                                          uint32_t a, uint32_t b, int k, Xchg& X, bool act) {
   xchg_any(v, L, N, L.thr_l, N.thr_l, a, b, k, X, act);
+}
+
+// ---- layout swaps ------------------------------------------------------------
+// A layout op is a list of swaps (see the planner's TK_LAYOUT packing). Swap
+// (slot j, thread index bit m): the local bits of register slot j and of
+// thread bit m trade places. Per register pair x = v[r] (bit j of r clear),
+// y = v[r | 2^j], a thread whose bit m is 0 keeps x and receives its
+// partner's (thread ^ 2^m) x into y; a thread whose bit m is 1 keeps y and
+// receives its partner's y into x. Half the amplitudes move.
+
+// 32-bit halves of a double in place
+struct U2 {
+  uint32_t lo, hi;
+};
+__device__ __forceinline__ U2 split(double d) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, d);
+  return {(uint32_t)u, (uint32_t)(u >> 32)};
+}
+__device__ __forceinline__ double join(U2 h) {
+  return __builtin_bit_cast(double, ((uint64_t)h.hi << 32) | h.lo);
+}
+
+// Thread bits m < 4 within a wave: DPP-sourced selects. tm = lanes whose
+// bit m is 1. y' = tm ? y : x[l + 2^m]; x' = tm ? y_old[l - 2^m] : x. Lanes
+// whose DPP source lies outside their row are not written, and those are
+// exactly lanes that keep their value.
+#define QOA_DPP_SWAP(UP, DOWN)                                                              \
+  asm("s_mov_b64 vcc, %[tm]\n\t"                                                            \
+      "v_mov_b32 %[t0], %[y0]\n\t"                                                          \
+      "v_mov_b32 %[t1], %[y1]\n\t"                                                          \
+      "v_mov_b32 %[t2], %[y2]\n\t"                                                          \
+      "v_mov_b32 %[t3], %[y3]\n\t"                                                          \
+      "s_nop 1\n\t"                                                                         \
+      "v_cndmask_b32_dpp %[y0], %[x0], %[y0], vcc " UP " row_mask:0xf bank_mask:0xf\n\t"    \
+      "v_cndmask_b32_dpp %[y1], %[x1], %[y1], vcc " UP " row_mask:0xf bank_mask:0xf\n\t"    \
+      "v_cndmask_b32_dpp %[y2], %[x2], %[y2], vcc " UP " row_mask:0xf bank_mask:0xf\n\t"    \
+      "v_cndmask_b32_dpp %[y3], %[x3], %[y3], vcc " UP " row_mask:0xf bank_mask:0xf\n\t"    \
+      "s_not_b64 vcc, vcc\n\t"                                                              \
+      "s_nop 1\n\t"                                                                         \
+      "v_cndmask_b32_dpp %[x0], %[t0], %[x0], vcc " DOWN " row_mask:0xf bank_mask:0xf\n\t"  \
+      "v_cndmask_b32_dpp %[x1], %[t1], %[x1], vcc " DOWN " row_mask:0xf bank_mask:0xf\n\t"  \
+      "v_cndmask_b32_dpp %[x2], %[t2], %[x2], vcc " DOWN " row_mask:0xf bank_mask:0xf\n\t"  \
+      "v_cndmask_b32_dpp %[x3], %[t3], %[x3], vcc " DOWN " row_mask:0xf bank_mask:0xf"      \
+      : [x0] "+v"(a0.lo), [x1] "+v"(a0.hi), [x2] "+v"(a1.lo), [x3] "+v"(a1.hi),             \
+        [y0] "+v"(b0.lo), [y1] "+v"(b0.hi), [y2] "+v"(b1.lo), [y3] "+v"(b1.hi),             \
+        [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)                      \
+      : [tm] "s"(tm)                                                                        \
+      : "vcc")
+
+template <int M>
+__device__ __forceinline__ void lane_swap_pair(double2& x, double2& y) {
+  U2 a0 = split(x.x), a1 = split(x.y), b0 = split(y.x), b1 = split(y.y);
+  if constexpr (M >= 4) {  // rows 16 / halves 32 apart: x's upper part <-> y's lower part
+    asm("s_nop 1\n\t"
+        "v_permlane%c[w]_swap_b32 %[x0], %[y0]\n\t"
+        "v_permlane%c[w]_swap_b32 %[x1], %[y1]\n\t"
+        "v_permlane%c[w]_swap_b32 %[x2], %[y2]\n\t"
+        "v_permlane%c[w]_swap_b32 %[x3], %[y3]"
+        : [x0] "+v"(a0.lo), [x1] "+v"(a0.hi), [x2] "+v"(a1.lo), [x3] "+v"(a1.hi),
+          [y0] "+v"(b0.lo), [y1] "+v"(b0.hi), [y2] "+v"(b1.lo), [y3] "+v"(b1.hi)
+        : [w] "i"(M == 4 ? 16 : 32));
+  } else {
+    constexpr uint64_t kMask[4] = {0xAAAAAAAAAAAAAAAAull, 0xCCCCCCCCCCCCCCCCull,
+                                   0xF0F0F0F0F0F0F0F0ull, 0xFF00FF00FF00FF00ull};
+    const uint64_t tm = kMask[M];
+    uint32_t t0, t1, t2, t3;
+    if constexpr (M == 0) QOA_DPP_SWAP("quad_perm:[1,0,3,2]", "quad_perm:[1,0,3,2]");
+    else if constexpr (M == 1) QOA_DPP_SWAP("quad_perm:[2,3,0,1]", "quad_perm:[2,3,0,1]");
+    else if constexpr (M == 2) QOA_DPP_SWAP("row_shl:4", "row_shr:4");
+    else QOA_DPP_SWAP("row_shl:8", "row_shr:8");
+  }
+  x = make_double2(join(a0), join(a1));
+  y = make_double2(join(b0), join(b1));
+}
+#undef QOA_DPP_SWAP
+
+template <int J, int M>
+__device__ __forceinline__ void lane_swap(double2 (&v)[kR]) {
+#pragma unroll
+  for (int r = 0; r < kR; ++r)
+    if (!(r & (1 << J))) lane_swap_pair<M>(v[r], v[r | (1 << J)]);
+}
+
+template <int J>
+__device__ __forceinline__ void lane_swap_m(double2 (&v)[kR], uint32_t m) {
+  switch (m) {
+    case 0: lane_swap<J, 0>(v); break;
+    case 1: lane_swap<J, 1>(v); break;
+    case 2: lane_swap<J, 2>(v); break;
+    case 3: lane_swap<J, 3>(v); break;
+    case 4: lane_swap<J, 4>(v); break;
+    default: lane_swap<J, 5>(v); break;
+  }
+}
+
+// Thread bits m >= 6 (between waves; bit m is the same in a whole wave):
+// through LDS, element pair * T + thread (T = threads of the tile), with
+// barriers before (earlier reads of the area), between and after. Every
+// wave passes the same three barriers; the branches between them are
+// wave-uniform.
+template <int J>
+__device__ __forceinline__ void wave_swap(double2 (&v)[kR], uint32_t m, uint32_t T, double2* lds,
+                                          bool act) {
+  const uint32_t tid = threadIdx.x;
+  const bool hi = __builtin_amdgcn_readfirstlane((tid >> m) & 1u) != 0u;
+  const uint32_t src = tid ^ (1u << m);
+  __syncthreads();
+  if (act) {
+    if (hi) {  // keeps y = v[r | 2^J], sends x
+#pragma unroll
+      for (int r = 0, p = 0; r < kR; ++r)
+        if (!(r & (1 << J))) lds[(uint32_t)(p++) * T + tid] = v[r];
+    } else {  // keeps x, sends y
+#pragma unroll
+      for (int r = 0, p = 0; r < kR; ++r)
+        if (!(r & (1 << J))) lds[(uint32_t)(p++) * T + tid] = v[r | (1 << J)];
+    }
+  }
+  __syncthreads();
+  if (act) {
+    if (hi) {
+#pragma unroll
+      for (int r = 0, p = 0; r < kR; ++r)
+        if (!(r & (1 << J))) v[r] = lds[(uint32_t)(p++) * T + src];
+    } else {
+#pragma unroll
+      for (int r = 0, p = 0; r < kR; ++r)
+        if (!(r & (1 << J))) v[r | (1 << J)] = lds[(uint32_t)(p++) * T + src];
+    }
+  }
+  __syncthreads();
+}
+
+// Register slots i <-> j: a renaming of the registers.
+template <int I, int J>
+__device__ __forceinline__ void slot_swap(double2 (&v)[kR]) {
+#pragma unroll
+  for (int r = 0; r < kR; ++r)
+    if ((r & (1 << I)) && !(r & (1 << J))) ip_swap2(v[r], v[r ^ (1 << I) ^ (1 << J)]);
+}
+
+// Applies the swap list of a layout op; tbp packs the thread map (4 bits per
+// thread index bit: its local bit).
+__device__ __forceinline__ void apply_swaps(double2 (&v)[kR], RegLayout& L, uint64_t& tbp,
+                                            uint32_t sw, int k, double2* lds, bool act) {
+  const uint32_t n = sw >> 28;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t e = (sw >> (7 * i)) & 0x7Fu;
+    const uint32_t j = e & 3u, opd = (e >> 2) & 15u;
+    if (e & 64u) {  // slot j <-> slot opd
+      const uint32_t lo = j < opd ? j : opd, hi = j < opd ? opd : j;
+      switch (lo * 4 + hi) {
+        case 1: slot_swap<1, 0>(v); break;
+        case 2: slot_swap<2, 0>(v); break;
+        case 3: slot_swap<3, 0>(v); break;
+        case 6: slot_swap<2, 1>(v); break;
+        case 7: slot_swap<3, 1>(v); break;
+        default: slot_swap<3, 2>(v); break;
+      }
+      const uint32_t t = L.rb[j];
+      L.rb[j] = L.rb[opd];
+      L.rb[opd] = t;
+      continue;
+    }
+    // slot j <-> thread bit m runs through slot 3: slots j and 3 trade their
+    // registers first (slot j then holds slot 3's bit), then slot 3 and
+    // thread bit m trade places
+    const uint32_t m = opd;
+    switch (j) {
+      case 0: slot_swap<3, 0>(v); break;
+      case 1: slot_swap<3, 1>(v); break;
+      case 2: slot_swap<3, 2>(v); break;
+      default: break;
+    }
+    const uint32_t t3 = L.rb[3];
+    L.rb[3] = L.rb[j];
+    L.rb[j] = t3;
+    const uint32_t pb = __builtin_ctz(L.rb[3]);                // leaves the registers
+    const uint32_t qb = (uint32_t)(tbp >> (4 * m)) & 15u;      // enters them
+    if (m < 6) lane_swap_m<3>(v, m);
+    else wave_swap<3>(v, m, 1u << (k - kRegBits), lds, act);
+    L.thr_l = (L.thr_l & ~(1u << qb)) | (((tid >> m) & 1u) << pb);
+    L.rb[3] = 1u << qb;
+    tbp = (tbp & ~(15ull << (4 * m))) | ((uint64_t)pb << (4 * m));
+  }
 }
 
 template <int J, bool Y>
@@ -1022,8 +1209,10 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const
 // (its phase, a tile constant, is applied at once), and the frame bits that
 // fall inside the tile are applied when the tile is loaded. Returns the frame
 // leaving the pass. NOISE = false compiles the noise handling out (nz.on
-// must then be 0). Waves without active threads skip the arithmetic.
-template <typename I, bool F, bool NOISE, bool SCALAR_IDLE>
+// must then be 0). Waves without active threads skip the arithmetic. SWAPS:
+// the kernel takes layout ops that are swap lists (plans made with
+// kPlanSwaps); without it every layout op is a whole-tile exchange.
+template <typename I, bool F, bool NOISE, bool SCALAR_IDLE, bool SWAPS>
 __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, const Geo& geo,
                                              const TileOp* __restrict__ ops, int nops,
                                              const double* __restrict__ pool, double2* lds,
@@ -1035,6 +1224,8 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
   for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(k - kRegBits + j);
   RegLayout L;
   set_layout(L, tid, k, rpos);  // thr_l = tid: thread bits are local bits 0..k-5
+  uint64_t tbp = 0;             // thread index bit m stands for local bit (tbp >> 4m) & 15
+  for (int m = 0; m < k - kRegBits; ++m) tbp |= (uint64_t)m << (4 * m);
   // HBM addresses: base | thread part | register part (uniform per r)
   I gthr = 0;
   for (int j = 0; j < k - kRegBits; ++j) gthr |= (I)((tid >> j) & 1u) << geo.spos[j];
@@ -1096,13 +1287,26 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     OpRegs next;
     if (NOISE) op = load_op(ops + o);
     else next = load_op(ops + (o + 1 < nops ? o + 1 : o));
-    if (op.kind() == TK_LAYOUT) {  // new slot bits in ins, two unchanged slots in t, t2
+    if (op.kind() == TK_LAYOUT) {  // the new slot bits in ins
 #pragma unroll
       for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(op.w[1] >> (8 * j));
-      RegLayout N;
-      set_layout(N, tid, k, rpos);
-      relayout(v, L, N, (op.w[0] >> 8) & 0xFFu, (op.w[0] >> 16) & 0xFFu, k, X, act);
-      L = N;
+      const uint32_t sw = op.aux();
+      if (SWAPS && (sw >> 31)) {  // swap list
+        apply_swaps(v, L, tbp, sw & 0x7FFFFFFFu, k, lds, act);
+      } else {  // whole-tile exchange; two unchanged slots in t, t2
+        RegLayout N;
+        set_layout(N, tid, k, rpos);
+        relayout(v, L, N, (op.w[0] >> 8) & 0xFFu, (op.w[0] >> 16) & 0xFFu, k, X, act);
+        L = N;
+        if (SWAPS) {
+          uint32_t rmask = 0;
+#pragma unroll
+          for (int j = 0; j < kRegBits; ++j) rmask |= 1u << rpos[j];
+          tbp = 0;
+          for (int b = 0, m = 0; b < k; ++b)
+            if (!(rmask & (1u << b))) tbp |= (uint64_t)b << (4 * m++);
+        }
+      }
     } else if (wave_act && (((uint64_t)base ^ frame) & op.gmask()) == op.gval()) {  // uniform
       reg_op<F>(v, op, L, pool);
     }

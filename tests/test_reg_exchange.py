@@ -12,6 +12,7 @@ import pytest
 from tile_emulator import plan_program
 
 K_REG = 4
+PLAN_FORCE_REG, PLAN_SWAPS = 1 << 31, 1 << 29  # internal planner flags (planner.hpp)
 
 
 def swz(x):
@@ -130,34 +131,95 @@ def test_thread_bit_flip_rounds(k):
         assert out == want
 
 
-@pytest.mark.parametrize("tile", [8, 10, 11, 12])
-def test_planner_layout_ops_keep_split_slots(tile):
-    """A layout op keeps the bits of the slots its exchange is keyed by and
-    changes at most 4 - q slots; every pass ends with the lowest local bits
-    (up to three) outside the registers."""
+def swaps_of(op):
+    sw = int(op["aux"]) & 0x7FFFFFFF
+    out = []
+    for i in range(sw >> 28):
+        e = (sw >> (7 * i)) & 0x7F
+        out.append((e & 3, (e >> 2) & 15, bool(e & 64)))
+    return out
+
+
+def run_swaps(vals, k, slots, tb, swaps):
+    """The device's data movement for a swap list (registers of every thread
+    hold local indices). Slot j <-> thread bit m runs through slot 3: slots j
+    and 3 trade registers, then with s = 3 a thread whose bit m is 0 keeps
+    x = v[r] (bit s of r clear) and receives its partner's x into y =
+    v[r | 2^s]; a thread whose bit m is 1 keeps y and receives its partner's y
+    into x. Slot <-> slot renames registers."""
+    T = len(vals)
+    for j, opd, rr in swaps:
+        if not rr and j != 3:
+            run_swaps(vals, k, slots, tb, [(j, 3, True)])
+            j = 3
+        if rr:
+            i2 = opd
+            for t in range(T):
+                row = list(vals[t])
+                for r in range(16):
+                    if (r >> j) & 1 and not (r >> i2) & 1:
+                        r2 = r ^ (1 << j) ^ (1 << i2)
+                        row[r], row[r2] = row[r2], row[r]
+                vals[t] = row
+            slots[j], slots[i2] = slots[i2], slots[j]
+            continue
+        m = opd
+        old = [list(row) for row in vals]
+        for t in range(T):
+            partner = t ^ (1 << m)
+            for r in range(16):
+                if (r >> j) & 1:
+                    continue
+                r1 = r | (1 << j)
+                if (t >> m) & 1:
+                    vals[t][r] = old[partner][r1]
+                else:
+                    vals[t][r1] = old[partner][r]
+        slots[j], tb[m] = tb[m], slots[j]
+
+
+def expect(k, slots, tb):
+    T = 1 << (k - K_REG)
+    return [[sum(((t >> m) & 1) << tb[m] for m in range(k - K_REG)) |
+             sum(((r >> j) & 1) << slots[j] for j in range(K_REG)) for r in range(16)]
+            for t in range(T)]
+
+
+@pytest.mark.parametrize("tile", [8, 10, 11, 12, 13])
+def test_planner_swap_lists_move_the_tile(tile):
+    """Every layout op of a pass, applied with the device's data movement,
+    leaves each thread's registers holding the amplitudes the planner's slots
+    name; a pass ends with the lowest local bits (up to three) on the lowest
+    thread index bits."""
     from qoa_amd import workloads as W
     g = W.layered_random(16, 12, seed=5, ladders=True)
-    ops, spos, rng, k = plan_program(16, g, tile_qubits=tile)
-    q = split(k)
+    # the batched kernel's plans (swaps for tiles of up to 10 qubits)
+    ops, spos, rng, k = plan_program(16, g, tile_qubits=tile, flags=PLAN_SWAPS | PLAN_FORCE_REG)
     seen = 0
+    swaps = 0
     for b0, b1, reg in rng:
         if not reg:
             continue
-        layout = list(range(k - K_REG, k))
+        slots = list(range(k - K_REG, k))
+        tb = list(range(k - K_REG))
+        vals = expect(k, slots, tb)
         for op in ops[b0:b1]:
             if int(op["kind"]) != 9:
                 continue
             seen += 1
-            new = [int(x) for x in op["ins"]]
-            a, b = int(op["t"]), int(op["t2"])
-            assert a < b and len(set(new)) == K_REG
-            for s in [a, b][:q]:
-                assert layout[s] == new[s]
-            assert sum(x != y for x, y in zip(layout, new)) <= K_REG - q
-            layout = new
-        low = (1 << min(3, k - K_REG)) - 1
-        assert not any(low >> b & 1 for b in layout)
+            if not int(op["aux"]) >> 31:  # whole-tile exchange (modelled above)
+                slots = [int(x) for x in op["ins"]]
+                tb = [b for b in range(k) if b not in slots]
+                vals = expect(k, slots, tb)
+                continue
+            run_swaps(vals, k, slots, tb, swaps_of(op))
+            swaps += 1
+            assert slots == [int(x) for x in op["ins"]]
+            assert vals == expect(k, slots, tb)
+        nlow = min(3, k - K_REG)
+        assert sorted(tb[:nlow]) == list(range(nlow))
     assert seen > 0
+    assert (swaps > 0) == (k <= 10)
 
 
 def test_planner_mat4_slots():

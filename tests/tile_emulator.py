@@ -91,11 +91,32 @@ def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int) -> None:
                 kind = int(op["kind"])
                 if kind == TK_LAYOUT:
                     new = [int(x) for x in op["ins"]]
-                    # the first q of slots (t, t2) keep their bit: the device
-                    # exchanges the tile in 2^q rounds keyed by those bits
-                    q = 0 if k <= 10 else 1 if k == 11 else 2
-                    for ks in [int(op["t"]), int(op["t2"])][:q]:
-                        assert layout[ks] == new[ks], "kept slots"
+                    sw = int(op["aux"])
+                    if not sw >> 31:
+                        # whole-tile exchange: the first q of slots (t, t2)
+                        # keep their bit (the rounds are keyed by those bits)
+                        q = 0 if k <= 10 else 1 if k == 11 else 2
+                        for ks in [int(op["t"]), int(op["t2"])][:q]:
+                            assert layout[ks] == new[ks], "kept slots"
+                        assert sorted(new) == sorted(set(new)), "slots hold distinct bits"
+                        layout = new
+                        continue
+                    # a list of swaps: slot <-> thread index bit (a bit from
+                    # outside the registers enters the slot) or slot <-> slot
+                    sw &= 0x7FFFFFFF
+                    cur = list(layout)
+                    for i in range(sw >> 28):
+                        e = (sw >> (7 * i)) & 0x7F
+                        j, opd = e & 3, (e >> 2) & 15
+                        if e & 64:
+                            cur[j], cur[opd] = cur[opd], cur[j]
+                        else:  # through slot 3: slot j takes slot 3's bit
+                            cur[j], cur[3] = cur[3], None
+                    for j in range(4):
+                        if cur[j] is None:
+                            assert new[j] not in layout, "a swapped-in bit was a register bit"
+                        else:
+                            assert new[j] == cur[j], "slot changed without a swap"
                     assert sorted(new) == sorted(set(new)), "slots hold distinct bits"
                     layout = new
                     continue
