@@ -42,6 +42,16 @@ COMMON = [
     "-I", str(PKG.parent / "include"),
 ]
 
+# A kernel whose code fits in +-128 KiB needs no long jumps (s_getpc /
+# s_add / s_addc / s_setpc for one branch), but the compiler sizes each inline
+# asm statement at 16 bytes, about twice the real size of the register
+# kernels' arithmetic, and turns branches it believes far into long jumps
+# (128 of them in the fused register kernel, most inside its op loop). Telling
+# it the branch field reaches twice as far removes them; the assembler still
+# rejects any branch whose real offset does not fit ("branch size exceeds
+# simm16"), and such a source is compiled again without the option.
+NEAR_BRANCHES = ["-mllvm", "-amdgpu-s-branch-bits=17"]
+
 
 def sources() -> list[Path]:
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
@@ -49,8 +59,11 @@ def sources() -> list[Path]:
 
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON, "-x", "hip", "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, *NEAR_BRANCHES, "-x", "hip", "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0 and "branch size exceeds simm16" in res.stderr:
+        cmd = [HIPCC, *COMMON, "-x", "hip", "-c", str(src), "-o", str(obj)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stdout}\n{res.stderr}")
     return obj
