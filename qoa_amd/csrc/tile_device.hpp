@@ -1125,6 +1125,23 @@ __device__ __forceinline__ OpRegs load_op(const TileOp* __restrict__ p) {
   return r;
 }
 
+// The words an op's dispatch reads (kind, slots, tile condition, controls)
+// and its coefficient words 8..23, which only the arithmetic after dispatch
+// reads.
+constexpr int kCoefFirst = 8, kCoefEnd = 24;
+__device__ __forceinline__ bool head_word(int i) { return i < kCoefFirst || i >= kCoefEnd; }
+__device__ __forceinline__ void load_head(OpRegs& r, const TileOp* __restrict__ p) {
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+  for (int i = 0; i < kOpWords; ++i)
+    if (head_word(i)) r.w[i] = q[i];
+}
+__device__ __forceinline__ void load_coefs(OpRegs& r, const TileOp* __restrict__ p) {
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+  for (int i = kCoefFirst; i < kCoefEnd; ++i) r.w[i] = q[i];
+}
+
 // Applies one non-layout op whose tile condition holds. tok: this thread's
 // control bits (tl) are all set; lanes without it keep their amplitudes
 // (for diagonal ops through a factor of exactly 1). It is computed only on
@@ -1272,21 +1289,26 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     }
   }
 
-  // Without noise the next op's words are loaded while the current op runs;
-  // the noise handling needs those scalar registers, so with noise each op
-  // is loaded at the top of its iteration.
+  // Without noise the next op's dispatch words are loaded while the current
+  // op runs, and the current op's coefficients at the top of its iteration
+  // (they are read after dispatch): only the dispatch words pass from one
+  // iteration to the next. The noise handling needs those scalar registers,
+  // so with noise each op is loaded at the top of its iteration.
   OpRegs op;
   if (!NOISE && nops > 0) {
-    op = load_op(ops);
+    load_head(op, ops);
     // Consume the first op's words here: the wait for them then sits before
-    // the loop, and the loop's only wait on scalar loads is the one for the
-    // prefetched op at the end of each iteration.
+    // the loop.
     asm volatile("" ::"s"(op.w[0]));
   }
   for (int o = 0; o < nops; ++o) {
     OpRegs next;
-    if (NOISE) op = load_op(ops + o);
-    else next = load_op(ops + (o + 1 < nops ? o + 1 : o));
+    if (NOISE) {
+      op = load_op(ops + o);
+    } else {
+      load_head(next, ops + (o + 1 < nops ? o + 1 : o));
+      load_coefs(op, ops + o);
+    }
     if (op.kind() == TK_LAYOUT) {  // the new slot bits in ins
 #pragma unroll
       for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(op.w[1] >> (8 * j));
@@ -1339,7 +1361,11 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         }
       }
     }
-    if (!NOISE) op = next;
+    if (!NOISE) {
+#pragma unroll
+      for (int i = 0; i < kOpWords; ++i)
+        if (head_word(i)) op.w[i] = next.w[i];
+    }
   }
   if (act) {  // store from the final layout
     gthr = 0;
