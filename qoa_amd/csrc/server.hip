@@ -497,7 +497,7 @@ int qoa_server_create(const qoa_server_opts* opts, qoa_server** out) {
             hipHostMalloc((void**)&s->pinned_pool, sizeof(uint64_t) * kMaxTraj * o.max_jobs,
                           hipHostMallocDefault) == hipSuccess &&
             hipMalloc(&s->arena_base, o.budget_bytes) == hipSuccess &&
-            hipMalloc(&s->d_ops, sizeof(TileOp) * (uint64_t)o.max_ops) == hipSuccess &&
+            hipMalloc(&s->d_ops, sizeof(TileOp) * ((uint64_t)o.max_ops + 1)) == hipSuccess &&
             hipMalloc(&s->d_frames, sizeof(uint64_t) * (uint64_t)kFrameBlock * o.max_jobs) ==
                 hipSuccess;
   for (uint32_t i = 0; ok && i < kRing; ++i) {

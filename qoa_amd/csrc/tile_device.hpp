@@ -1294,6 +1294,8 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
   // (they are read after dispatch): only the dispatch words pass from one
   // iteration to the next. The noise handling needs those scalar registers,
   // so with noise each op is loaded at the top of its iteration.
+  // The op after the last one is read too (its words are not used): every
+  // op buffer ends with one spare op.
   OpRegs op;
   if (!NOISE && nops > 0) {
     load_head(op, ops);
@@ -1301,13 +1303,14 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     // the loop.
     asm volatile("" ::"s"(op.w[0]));
   }
-  for (int o = 0; o < nops; ++o) {
+  const TileOp* const ops_end = ops + nops;
+  for (const TileOp* p = ops; p != ops_end; ++p) {
     OpRegs next;
     if (NOISE) {
-      op = load_op(ops + o);
+      op = load_op(p);
     } else {
-      load_head(next, ops + (o + 1 < nops ? o + 1 : o));
-      load_coefs(op, ops + o);
+      load_head(next, p + 1);
+      load_coefs(op, p);
     }
     if (op.kind() == TK_LAYOUT) {  // the new slot bits in ins
 #pragma unroll
