@@ -39,10 +39,11 @@ static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
 // NOISE = false: no item of the launch has trajectory noise. SWAPS: the
 // items' plans may hold swap-list layout ops (tiles of up to 10 qubits,
 // planned with kPlanSwaps); plans of larger tiles hold whole-tile exchanges
-// only, which either instantiation runs. At most 168 VGPRs: three waves per
-// SIMD.
+// only, which either instantiation runs. With noise or swaps at most 168
+// VGPRs (three waves per SIMD); the noise-free swap-free kernel, whose blocks
+// have no idle lanes, at most 128 (it takes 96: five waves per SIMD).
 template <typename I, bool NOISE, bool SWAPS>
-__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(NOISE || SWAPS ? 3 : 4)))
 void k_batch_reg(double2* __restrict__ states,
                                                             const WorkItem* __restrict__ items,
                                                             int nitems,
@@ -66,7 +67,10 @@ void k_batch_reg(double2* __restrict__ states,
     const int b = geo.spos[j];
     base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
   }
-  const bool act = threadIdx.x < (1u << (it.k - kRegBits));
+  // Groups of 11+-qubit tiles (the only ones the swap-free kernel runs) have
+  // exactly 2^(k-4) threads per block; one-wave blocks of smaller tiles have
+  // idle lanes.
+  const bool act = !SWAPS || threadIdx.x < (1u << (it.k - kRegBits));
   NoiseArgs nz;
   nz.on = it.noise;
   nz.seed = it.seed;
