@@ -32,10 +32,11 @@ constexpr uint8_t kM2General = 0, kM2Real = 1, kM2Rx = 2;
 // lower local bit is in slot 2 and its higher one in slot 3.
 constexpr int kRegBits = 4;
 constexpr int kRegAmps = 1 << kRegBits;
-// Register passes exchange amplitudes through two LDS buffers of 2^(k-q)
-// elements, q = reg_xchg_split(k) (the exchange then takes 2^q rounds): up
-// to k = 12 a buffer holds at most 1024 elements (16 KiB).
-constexpr uint32_t reg_xchg_split(uint32_t k) { return k <= 10 ? 0u : k == 11 ? 1u : 2u; }
+// Register passes exchange amplitudes through one LDS buffer of 2^(k-q)
+// elements, q = reg_xchg_split(k) (the exchange then takes 2^q rounds, and
+// an exchange changes at most kRegBits - q register slots): the buffer holds
+// at most 2048 elements (32 KiB).
+constexpr uint32_t reg_xchg_split(uint32_t k) { return k <= 11 ? 0u : k == 12 ? 1u : 2u; }
 constexpr uint32_t reg_xchg_elems(uint32_t k) { return 1u << (k - reg_xchg_split(k)); }
 // PlanOptions.flags bit (internal): register mode for every tile of at least
 // kRegBits qubits (the batched kernel runs small whole-state tiles this way).

@@ -684,19 +684,19 @@ __device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u) {
 }
 
 // ---- exchanges through LDS --------------------------------------------------
-// Two buffers of 2^(k-Q) amplitudes, indexed through swz (bits 4..7 of an
+// One buffer of 2^(k-Q) amplitudes, indexed through swz (bits 4..7 of an
 // index XORed into bits 0..3, which spreads the strided layouts over the
-// banks without padding), are used alternately: a round writes one buffer, waits at
-// a barrier and reads it back, and the next round's writes go to the other
-// buffer, so one barrier per round suffices. Q = reg_xchg_split(k) register
-// slots (A, then B), whose bits are the same before and after the exchange,
-// split it into 2^Q rounds: round c moves the amplitudes whose bits at those
-// slots read c, and a buffer index is the local index without those bits.
-// Every thread of the block calls these (act == false: barriers only).
+// banks without padding). Q = reg_xchg_split(k) register slots (A, then B),
+// whose bits are the same before and after the exchange, split it into 2^Q
+// rounds: round c moves the amplitudes whose bits at those slots read c, and
+// a buffer index is the local index without those bits. A round waits at a
+// barrier (earlier reads of the buffer are done), writes, waits at a second
+// barrier and reads back. (At k = 12 that is 4 barriers, as many as four
+// double-buffered rounds of 1024 amplitudes took, while one slot fewer has
+// to stay in place.) Every thread of the block calls these (act == false:
+// barriers only).
 struct Xchg {
   double2* lds;
-  uint32_t stride;  // elements per buffer
-  uint32_t round;
 };
 
 // local index l without bit p
@@ -755,9 +755,10 @@ __device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const
   slot_offsets<Q, A, B>(L, pa, pb, ow);
   slot_offsets<Q, A, B>(N, pa, pb, orr);
   const uint32_t bw = swz(squeeze<Q>(tw, pa, pb)), br = swz(squeeze<Q>(tr, pa, pb));
+  double2* buf = X.lds;
 #pragma unroll
   for (int c = 0; c < (1 << Q); ++c) {
-    double2* buf = X.lds + (X.round & 1u) * X.stride;
+    __syncthreads();
     if (act) {
 #pragma unroll
       for (int r = 0; r < kR; ++r)
@@ -769,7 +770,6 @@ __device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const
       for (int r = 0; r < kR; ++r)
         if (round_of<Q, A, B>(r) == c) v[r] = buf[br ^ reg_offset<Q, A, B>(orr, r)];
     }
-    ++X.round;
   }
 }
 
@@ -1266,7 +1266,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
   }
   // the tile's loads complete here, so the op loop carries no waits for them
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt, lgkmcnt not waited for
-  Xchg X{lds, reg_xchg_elems((uint32_t)k), 0u};
+  Xchg X{lds};
   uint64_t frame = (NOISE && nz.on) ? nz.frame : 0ULL;
   // Threads are numbered wave by wave, so a wave is idle iff its first lane
   // is. SCALAR_IDLE: test that once per wave (a scalar branch; batched

@@ -1,5 +1,6 @@
 """Index model of the register kernels' LDS exchanges (CPU): the 2^q rounds
-(q = 0, 1, 2 by tile size) of a layout change and of thread-bit X flips,
+(q = 0, 1, 2 by tile size) of a layout change and of thread-bit X flips
+through one buffer,
 with the buffer indexing written as the device code computes it (squeeze,
 the XOR swizzle, per-thread part XOR per-register part). Every write lands inside its buffer, no two
 writes of a round collide, every read finds a value written in the same
@@ -26,7 +27,7 @@ def squeeze(l, bits):
 
 
 def split(k):
-    return 0 if k <= 10 else 1 if k == 11 else 2
+    return 0 if k <= 11 else 1 if k == 12 else 2
 
 
 def stride(k):
@@ -69,10 +70,11 @@ def xchg(vals, k, rold, rnew, tw_of, tr_of, A, B):
         return x
     rnd = lambda r: sum(((r >> j) & 1) << i for i, j in enumerate(sl))
     S = stride(k)
-    lds = [None] * (2 * S)
+    assert S <= 2048  # 32 KiB
     out = [list(v) for v in vals]
     for c in range(1 << Q):
-        base = (c & 1) * S
+        lds = [None] * S  # a round's reads see only that round's writes
+        base = 0
         written = set()
         for t in range(T):
             bw = swz(squeeze(tw_of(t), bits))
@@ -100,7 +102,7 @@ def tile_regs(k, rpos):
     return [[set_layout(t, k, rpos) | regl(rpos, r) for r in range(1 << K_REG)] for t in range(T)]
 
 
-@pytest.mark.parametrize("k", [5, 6, 8, 10, 11, 12])
+@pytest.mark.parametrize("k", [5, 6, 8, 10, 11, 12, 13])
 def test_layout_change_rounds(k):
     rng = np.random.default_rng(k)
     for _ in range(40 if k < 11 else 8):

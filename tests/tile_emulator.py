@@ -95,7 +95,7 @@ def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int) -> None:
                     if not sw >> 31:
                         # whole-tile exchange: the first q of slots (t, t2)
                         # keep their bit (the rounds are keyed by those bits)
-                        q = 0 if k <= 10 else 1 if k == 11 else 2
+                        q = 0 if k <= 11 else 1 if k == 12 else 2
                         for ks in [int(op["t"]), int(op["t2"])][:q]:
                             assert layout[ks] == new[ks], "kept slots"
                         assert sorted(new) == sorted(set(new)), "slots hold distinct bits"
