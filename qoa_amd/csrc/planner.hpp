@@ -49,28 +49,31 @@ constexpr uint32_t kPlanSwaps = 1u << 29;
 
 // One op of a pass program. `ins` holds the sorted local bit positions into
 // which a zero is deposited to enumerate the visited indices (nins == 0xFF:
-// too many bits, scan every index with the predicate instead).
+// too many bits, scan every index with the predicate instead). The first
+// eight words are what the register kernels' op dispatch reads (they are
+// prefetched for the next op); the rest is read after dispatch.
 struct alignas(16) TileOp {
   uint8_t kind, t, t2, nins;
   uint8_t ins[4];
-  uint32_t lmask;
-  uint32_t aux;
-  uint64_t gmask, gval;  // op applies to a tile iff (tile_base & gmask) == gval
-  double c[8];
-  // Trajectory noise after this op (evaluated in every tile, whether or not
-  // the op's condition held there): for each of the nq operand qubits
-  // qphys[j] a Philox draw keyed by (seed, trajectory, gi, qphys[j]) picks
-  // X/Y/Z with probability p1 (psel == 1) or p2 (psel == 2).
-  uint32_t gi;
-  uint8_t nq, psel;
-  uint8_t qphys[3];
   // Register mode only (filled per layout by the segment scheduler): the
   // control bits of lmask split into register slots (rsel: slot j set means
   // local bit ins-of-layout[j] must be 1) and thread bits (tl = lmask & ~R);
   // rt is the register slot of a TK_RZ target, 0xFF when it is a thread bit.
   uint8_t rsel, rt;
   uint8_t outside;  // noise: the op reads or hits a qubit outside the tile
+  uint8_t nq;       // noise: operand qubits (below)
   uint32_t tl;
+  uint64_t gmask, gval;  // op applies to a tile iff (tile_base & gmask) == gval
+  uint32_t aux;
+  uint32_t lmask;
+  // Trajectory noise after this op (evaluated in every tile, whether or not
+  // the op's condition held there): for each of the nq operand qubits
+  // qphys[j] a Philox draw keyed by (seed, trajectory, gi, qphys[j]) picks
+  // X/Y/Z with probability p1 (psel == 1) or p2 (psel == 2).
+  uint32_t gi;
+  uint8_t psel;
+  uint8_t qphys[3];
+  double c[8];
 };
 static_assert(sizeof(TileOp) == 112, "TileOp layout");
 

@@ -1097,24 +1097,25 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
 // w26 = qphys[2] | rsel << 8 | rt << 16 | outside << 24, w27 = tl.
 constexpr int kOpWords = 28;
 static_assert(sizeof(TileOp) == 4 * kOpWords && offsetof(TileOp, ins) == 4 &&
-                  offsetof(TileOp, aux) == 12 && offsetof(TileOp, gmask) == 16 &&
-                  offsetof(TileOp, gval) == 24 && offsetof(TileOp, c) == 32 &&
-                  offsetof(TileOp, gi) == 96 && offsetof(TileOp, nq) == 100 &&
-                  offsetof(TileOp, qphys) == 102 && offsetof(TileOp, rsel) == 105 &&
-                  offsetof(TileOp, rt) == 106 && offsetof(TileOp, tl) == 108,
+                  offsetof(TileOp, rsel) == 8 && offsetof(TileOp, rt) == 9 &&
+                  offsetof(TileOp, nq) == 11 && offsetof(TileOp, tl) == 12 &&
+                  offsetof(TileOp, gmask) == 16 && offsetof(TileOp, gval) == 24 &&
+                  offsetof(TileOp, aux) == 32 && offsetof(TileOp, gi) == 40 &&
+                  offsetof(TileOp, psel) == 44 && offsetof(TileOp, qphys) == 45 &&
+                  offsetof(TileOp, c) == 48,
               "TileOp word layout");
 
 struct OpRegs {
   uint32_t w[kOpWords];
   __device__ __forceinline__ uint32_t kind() const { return w[0] & 0xFFu; }
-  __device__ __forceinline__ uint32_t aux() const { return w[3]; }
+  __device__ __forceinline__ uint32_t aux() const { return w[8]; }
   __device__ __forceinline__ uint64_t gmask() const { return w[4] | (uint64_t)w[5] << 32; }
   __device__ __forceinline__ uint64_t gval() const { return w[6] | (uint64_t)w[7] << 32; }
   __device__ __forceinline__ double c(int i) const {
-    return __builtin_bit_cast(double, w[8 + 2 * i] | (uint64_t)w[9 + 2 * i] << 32);
+    return __builtin_bit_cast(double, w[12 + 2 * i] | (uint64_t)w[13 + 2 * i] << 32);
   }
   __device__ __forceinline__ double2 c2(int i) const { return make_double2(c(2 * i), c(2 * i + 1)); }
-  __device__ __forceinline__ uint32_t gi() const { return w[24]; }
+  __device__ __forceinline__ uint32_t gi() const { return w[10]; }
 };
 
 __device__ __forceinline__ OpRegs load_op(const TileOp* __restrict__ p) {
@@ -1125,11 +1126,11 @@ __device__ __forceinline__ OpRegs load_op(const TileOp* __restrict__ p) {
   return r;
 }
 
-// The words an op's dispatch reads (kind, slots, tile condition, controls)
-// and its coefficient words 8..23, which only the arithmetic after dispatch
-// reads.
-constexpr int kCoefFirst = 8, kCoefEnd = 24;
-__device__ __forceinline__ bool head_word(int i) { return i < kCoefFirst || i >= kCoefEnd; }
+// The words an op's dispatch reads (kind, slots, controls, tile condition:
+// words 0..7) and the rest (the 4x4 pool offset or swap list, noise fields,
+// coefficients), which only the work after dispatch reads.
+constexpr int kCoefFirst = 8, kCoefEnd = kOpWords;
+__device__ __forceinline__ bool head_word(int i) { return i < kCoefFirst; }
 __device__ __forceinline__ void load_head(OpRegs& r, const TileOp* __restrict__ p) {
   const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
 #pragma unroll
@@ -1150,8 +1151,8 @@ template <bool F>
 __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const RegLayout& L,
                                        const double* __restrict__ pool) {
   const uint32_t kind = op.kind();
-  const uint32_t rsel = (op.w[26] >> 8) & 0xFFu;
-  const uint32_t tl = op.w[27];
+  const uint32_t rsel = op.w[2] & 0xFFu;
+  const uint32_t tl = op.w[3];
   auto tok = [&]() { return (L.thr_l & tl) == tl; };
   const uint32_t s0 = op.w[1] & 0xFFu, s1 = (op.w[1] >> 8) & 0xFFu;
   switch (kind) {
@@ -1187,7 +1188,7 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const
     case TK_RZ: {
       const double2 m00 = op.c2(0);
       const double2 m11 = op.c2(1);
-      switch ((op.w[26] >> 16) & 0xFFu) {  // register slot of the target, 0xFF: a thread bit
+      switch ((op.w[2] >> 8) & 0xFFu) {  // register slot of the target, 0xFF: a thread bit
         case 0: reg_rz<0, F>(v, m00, m11); break;
         case 1: reg_rz<1, F>(v, m00, m11); break;
         case 2: reg_rz<2, F>(v, m00, m11); break;
@@ -1335,12 +1336,12 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     } else if (wave_act && (((uint64_t)base ^ frame) & op.gmask()) == op.gval()) {  // uniform
       reg_op<F>(v, op, L, pool);
     }
-    const uint32_t nq = op.w[25] & 0xFFu;
+    const uint32_t nq = op.w[2] >> 24;
     if (NOISE && nz.on && nq) {  // trajectory errors after the gate, operand order
-      const double p = ((op.w[25] >> 8) & 0xFFu) == 1 ? nz.p1 : nz.p2;
+      const double p = (op.w[11] & 0xFFu) == 1 ? nz.p1 : nz.p2;
       for (uint32_t j = 0; j < nq; ++j) {
         const uint32_t q =
-            j == 0 ? (op.w[25] >> 16) & 0xFFu : j == 1 ? op.w[25] >> 24 : op.w[26] & 0xFFu;
+            j == 0 ? (op.w[11] >> 8) & 0xFFu : j == 1 ? (op.w[11] >> 16) & 0xFFu : op.w[11] >> 24;
         const uint32_t code = traj_pauli(nz.seed, nz.traj, op.gi(), q, p);
         if (!code) continue;
         int lb = -1;

@@ -18,10 +18,9 @@ from qoa_amd import _native as N
 from qoa_amd.state import gates_array
 
 TILE_OP = np.dtype([("kind", "u1"), ("t", "u1"), ("t2", "u1"), ("nins", "u1"), ("ins", "u1", 4),
-                    ("lmask", "<u4"), ("aux", "<u4"), ("gmask", "<u8"), ("gval", "<u8"),
-                    ("c", "<f8", 8), ("gi", "<u4"), ("nq", "u1"), ("psel", "u1"),
-                    ("qphys", "u1", 3), ("rsel", "u1"), ("rt", "u1"), ("outside", "u1"),
-                    ("tl", "<u4")])
+                    ("rsel", "u1"), ("rt", "u1"), ("outside", "u1"), ("nq", "u1"), ("tl", "<u4"),
+                    ("gmask", "<u8"), ("gval", "<u8"), ("aux", "<u4"), ("lmask", "<u4"),
+                    ("gi", "<u4"), ("psel", "u1"), ("qphys", "u1", 3), ("c", "<f8", 8)])
 TK_H, TK_M, TK_X, TK_Y, TK_DNEG, TK_DPHASE, TK_RZ, TK_SWAP, TK_MAT4, TK_LAYOUT = range(10)
 
 
