@@ -719,14 +719,15 @@ __device__ __forceinline__ uint32_t swz(uint32_t x) { return x ^ ((x >> 4) & 15u
 
 // swz and squeeze are linear over XOR, and disjoint bit sets OR as they XOR,
 // so a buffer index is a per-thread part XOR a per-register part (uniform,
-// the XOR of per-slot terms; the split slots contribute nothing).
+// the XOR of per-slot terms; the split slots contribute nothing). Both parts
+// are kept as byte offsets (index * 16), so an access address is one XOR.
 template <int Q, int A, int B>
 __device__ __forceinline__ void slot_offsets(const RegLayout& L, uint32_t pa, uint32_t pb,
                                              uint32_t (&o)[kRegBits]) {
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j) {
     const bool split = (Q >= 1 && j == A) || (Q == 2 && j == B);
-    o[j] = split ? 0u : swz(squeeze<Q>(L.rb[j], pa, pb));
+    o[j] = split ? 0u : swz(squeeze<Q>(L.rb[j], pa, pb)) << 4;
   }
 }
 
@@ -754,21 +755,23 @@ __device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const
   uint32_t ow[kRegBits], orr[kRegBits];
   slot_offsets<Q, A, B>(L, pa, pb, ow);
   slot_offsets<Q, A, B>(N, pa, pb, orr);
-  const uint32_t bw = swz(squeeze<Q>(tw, pa, pb)), br = swz(squeeze<Q>(tr, pa, pb));
-  double2* buf = X.lds;
+  const uint32_t bw = swz(squeeze<Q>(tw, pa, pb)) << 4, br = swz(squeeze<Q>(tr, pa, pb)) << 4;
+  char* buf = reinterpret_cast<char*>(X.lds);
 #pragma unroll
   for (int c = 0; c < (1 << Q); ++c) {
     __syncthreads();
     if (act) {
 #pragma unroll
       for (int r = 0; r < kR; ++r)
-        if (round_of<Q, A, B>(r) == c) buf[bw ^ reg_offset<Q, A, B>(ow, r)] = v[r];
+        if (round_of<Q, A, B>(r) == c)
+          *reinterpret_cast<double2*>(buf + (bw ^ reg_offset<Q, A, B>(ow, r))) = v[r];
     }
     __syncthreads();
     if (act) {
 #pragma unroll
       for (int r = 0; r < kR; ++r)
-        if (round_of<Q, A, B>(r) == c) v[r] = buf[br ^ reg_offset<Q, A, B>(orr, r)];
+        if (round_of<Q, A, B>(r) == c)
+          v[r] = *reinterpret_cast<const double2*>(buf + (br ^ reg_offset<Q, A, B>(orr, r)));
     }
   }
 }
