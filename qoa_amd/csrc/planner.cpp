@@ -302,9 +302,9 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
 // Each thread of the register-resident kernel holds the kRegAmps amplitudes
 // whose local indices differ in the register bits R. Ops that mix only bits
 // of R (and every diagonal op) run in registers; moving to a new R costs
-// TK_LAYOUT ops (LDS exchanges in quarters of the tile). The same greedy list scheduling as the
-// pass planner picks R: ops on disjoint bits commute, so the emitted order
-// stays a valid reordering. The pass starts in the layout that holds the top
+// TK_LAYOUT ops (LDS exchanges in 2^reg_xchg_split(k) rounds, or swap
+// lists). The same greedy list scheduling as the pass planner picks R: ops
+// on disjoint bits commute, so the emitted order stays a valid reordering. The pass starts in the layout that holds the top
 // kRegBits local bits in registers (coalesced HBM access) and ends in one
 // whose lowest local bits are thread bits.
 struct SegOp {
