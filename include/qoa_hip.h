@@ -206,8 +206,9 @@ int qoa_run_gates(qoa_state* s, const qoa_gate* gates, uint32_t n_gates,
 /* Fused execution: the gate list is planned into passes; each pass loads
    2^tile_qubits-amplitude tiles (the low `low_qubits` qubits plus qubits
    chosen per pass) into LDS once, applies every gate of the pass there and
-   writes the tile back. tile_qubits = 0 picks the default. Results equal
-   the per-gate run of the same gates in planned order (bit for bit). */
+   writes the tile back. tile_qubits = 0 picks the default (12; 11 for
+   states of 13..17 qubits). Results equal the per-gate run of the same
+   gates in planned order (bit for bit). */
 typedef struct qoa_fuse_opts {
   uint32_t tile_qubits;  /* k: 2^k amplitudes per tile (0 = default)     */
   uint32_t low_qubits;   /* m: qubits 0..m-1 always in the tile (0 = default) */
