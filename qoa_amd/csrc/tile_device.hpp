@@ -559,10 +559,16 @@ __device__ __forceinline__ void reg_pair_sel(double2 (&v)[kR], const Coef4& k, u
     if (r & (1 << J)) continue;
     if (((uint32_t)r & rsel) != rsel) continue;  // uniform
     const int r1 = r | (1 << J);
-    double2 o0, o1;
-    pair_math<KIND, F>(v[r], v[r1], k, o0, o1);
-    ip_select2(v[r], o0, mask);
-    ip_select2(v[r1], o1, mask);
+    if constexpr (KIND == TK_X) {  // a conditional swap: two selects per word
+      const double2 x = v[r], y = v[r1];
+      v[r] = tok ? y : x;
+      v[r1] = tok ? x : y;
+    } else {
+      double2 o0, o1;
+      pair_math<KIND, F>(v[r], v[r1], k, o0, o1);
+      ip_select2(v[r], o0, mask);
+      ip_select2(v[r1], o1, mask);
+    }
   }
 }
 
