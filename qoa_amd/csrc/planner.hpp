@@ -39,7 +39,8 @@ constexpr int kRegAmps = 1 << kRegBits;
 constexpr uint32_t reg_xchg_split(uint32_t k) { return k <= 11 ? 0u : k == 12 ? 1u : 2u; }
 constexpr uint32_t reg_xchg_elems(uint32_t k) { return 1u << (k - reg_xchg_split(k)); }
 // PlanOptions.flags bit (internal): register mode for every tile of at least
-// kRegBits qubits (the batched kernel runs small whole-state tiles this way).
+// kRegBits qubits (the batched kernel runs small whole-state tiles this way);
+// the default tile stays at 12 qubits for states of 13..17 qubits too.
 constexpr uint32_t kPlanForceReg = 1u << 31;
 // PlanOptions.flags bit (internal): attach trajectory-noise operands to ops.
 constexpr uint32_t kPlanNoise = 1u << 30;
