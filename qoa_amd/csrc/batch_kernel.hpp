@@ -41,7 +41,8 @@ static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
 // planned with kPlanSwaps); plans of larger tiles hold whole-tile exchanges
 // only, which either instantiation runs. With noise or swaps at most 168
 // VGPRs (three waves per SIMD); the noise-free swap-free kernel, whose blocks
-// have no idle lanes, at most 128 (it takes 96: five waves per SIMD).
+// have no idle lanes, at most 128 (it takes 108: four waves per SIMD, measured
+// 1% faster than a 96-VGPR five-wave build with two spilled registers).
 template <typename I, bool NOISE, bool SWAPS>
 __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(NOISE || SWAPS ? 3 : 4)))
 void k_batch_reg(double2* __restrict__ states,
