@@ -309,18 +309,19 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
 // whose lowest local bits are thread bits.
 struct SegOp {
   uint32_t act, all, mixb;
+  uint32_t ctl;  // controls of a 2x2 or SWAP op (cheaper when they are register bits)
 };
 
 inline SegOp seg_masks_local(const TileOp& op) {
   switch (op.kind) {
     case TK_H: case TK_M: case TK_X: case TK_Y:
-      return {1u << op.t, (1u << op.t) | op.lmask, 1u << op.t};
+      return {1u << op.t, (1u << op.t) | op.lmask, 1u << op.t, op.lmask};
     case TK_SWAP: case TK_MAT4: {
       const uint32_t a = (1u << op.t) | (1u << op.t2);
-      return {a, a | op.lmask, a};
+      return {a, a | op.lmask, a, op.kind == TK_SWAP ? op.lmask : 0u};
     }
-    case TK_RZ: return {0u, 1u << op.t, 0u};
-    default: return {0u, op.lmask, 0u};
+    case TK_RZ: return {0u, 1u << op.t, 0u, 0u};
+    default: return {0u, op.lmask, 0u, 0u};
   }
 }
 
@@ -346,8 +347,26 @@ int seg_sim(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint3
   return cnt;
 }
 
+// Ops of a sweep over R whose controls all lie in R (they then need no
+// per-lane selects).
+int seg_bonus(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint32_t R,
+              uint32_t allb) {
+  BlockerT<uint32_t> b;
+  int cnt = 0;
+  for (uint32_t i : rem) {
+    if (b.free(m[i].all, m[i].mixb) && (m[i].act & ~R) == 0) {
+      if (m[i].ctl && (m[i].ctl & ~R) == 0) ++cnt;
+    } else {
+      b.add(m[i].all, m[i].mixb);
+      if ((b.mix & allb) == allb) break;
+    }
+  }
+  return cnt;
+}
+
 // Greedy choice of the next register set: starting from `seed`, add the bit
-// (or pair of bits) that lets the most ops run. The set keeps at least `keep`
+// (or pair of bits) that lets the most ops run; among bits that tie, the one
+// that puts the most controls of those ops in registers. The set keeps at least `keep`
 // bits of the current set `cur` (a layout change then takes one exchange).
 uint32_t seg_choose(const std::vector<SegOp>& m, const std::vector<uint32_t>& rem, uint32_t seed,
                     uint32_t k, uint32_t allb, uint32_t cur, int keep) {
@@ -358,11 +377,13 @@ uint32_t seg_choose(const std::vector<SegOp>& m, const std::vector<uint32_t>& re
     return have >= keep || kRegBits - __builtin_popcount(X) >= keep - have;
   };
   while (__builtin_popcount(R) < kRegBits) {
-    int best = -1, bests = -1;
+    int best = -1, bests = -1, bestb = -1;
     for (uint32_t b = 0; b < k; ++b) {
       if ((R & (1u << b)) || !room(R | (1u << b))) continue;
       const int sc = seg_sim(m, rem, R | (1u << b), allb);
-      if (sc > bests) { bests = sc; best = (int)b; }
+      if (sc < bests) continue;
+      const int bo = seg_bonus(m, rem, R | (1u << b), allb);
+      if (sc > bests || bo > bestb) { bests = sc; bestb = bo; best = (int)b; }
     }
     if (bests <= score && __builtin_popcount(R) + 2 <= kRegBits) {
       int b1 = -1, b2 = -1, bp = score;
