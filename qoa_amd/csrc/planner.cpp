@@ -309,7 +309,7 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
 // whose lowest local bits are thread bits.
 struct SegOp {
   uint32_t act, all, mixb;
-  uint32_t ctl;  // controls of a 2x2 or SWAP op (cheaper when they are register bits)
+  uint32_t ctl;  // controls of a 2x2, SWAP or diagonal op (cheaper as register bits)
 };
 
 inline SegOp seg_masks_local(const TileOp& op) {
@@ -321,7 +321,7 @@ inline SegOp seg_masks_local(const TileOp& op) {
       return {a, a | op.lmask, a, op.kind == TK_SWAP ? op.lmask : 0u};
     }
     case TK_RZ: return {0u, 1u << op.t, 0u, 0u};
-    default: return {0u, op.lmask, 0u, 0u};
+    default: return {0u, op.lmask, 0u, op.lmask};
   }
 }
 
