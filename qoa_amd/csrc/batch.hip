@@ -52,6 +52,7 @@ struct qoa_batch {
   double* d_pool;
   std::vector<qoa::Round> rounds;
   uint64_t* frames;  // 2 X-frame slots per instance (noisy trajectories)
+  uint32_t* codes;   // noise draws: per instance one word per op of its job's plan
   bool noisy;
   double alg_bytes;  // algorithmic HBM bytes of one run
   hipEvent_t ev0, ev1;
@@ -70,6 +71,7 @@ static void batch_free(qoa_batch* b) {
   if (b->d_ops) hipFree(b->d_ops);
   if (b->d_pool) hipFree(b->d_pool);
   if (b->frames) hipFree(b->frames);
+  if (b->codes) hipFree(b->codes);
   if (b->states) hipFree(b->states);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
@@ -124,6 +126,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
   b->d_ops = nullptr;
   b->d_pool = nullptr;
   b->frames = nullptr;
+  b->codes = nullptr;
   b->noisy = false;
   b->ev0 = b->ev1 = nullptr;
   b->last_ms = 0.f;
@@ -214,6 +217,27 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
       hipMemcpy(b->d_pool, all_pool.data(), sizeof(double) * all_pool.size(),
                 hipMemcpyHostToDevice) != hipSuccess)
     rc = set_error(QOA_ERR_HIP, "pool upload failed");
+  // noise draws: instance ii's words start at codes_base[ii] (a job's
+  // trajectories are consecutive instances, so its table is t * nops + o)
+  std::vector<uint64_t> codes_base(b->inst.size(), 0);
+  if (rc == QOA_OK && b->noisy) {
+    uint64_t words = 0;
+    for (size_t ii = 0; ii < b->inst.size(); ++ii) {
+      codes_base[ii] = words;
+      words += b->plans[b->inst[ii].job].ops.size();
+    }
+    if (hipMalloc(&b->codes, sizeof(uint32_t) * (words + 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = set_error(QOA_ERR_OOM, "noise draws");
+    }
+    for (uint32_t j = 0; rc == QOA_OK && j < n_jobs; ++j) {
+      const qoa_job_desc& jd = b->jobs[j];
+      if (!(jd.p1 > 0.0 || jd.p2 > 0.0)) continue;
+      draw_codes(b->stream, b->d_ops + ops_base[j], (uint32_t)b->plans[j].ops.size(),
+                 jd.noise_seed, jd.p1, jd.p2, jd.trajectories, b->codes + codes_base[b->first_inst[j]]);
+      if (hipGetLastError() != hipSuccess) rc = set_error(QOA_ERR_HIP, "noise draw launch failed");
+    }
+  }
   // rounds
   uint32_t max_passes = 0;
   for (auto& p : b->plans) max_passes = std::max<uint32_t>(max_passes, (uint32_t)p.passes.size());
@@ -231,9 +255,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
       WorkItem w{};
       w.state_off = in.off;
       w.noise = (jd.p1 > 0.0 || jd.p2 > 0.0) ? 1 : 0;
-      w.seed = jd.noise_seed;
-      w.p1 = jd.p1;
-      w.p2 = jd.p2;
+      w.codes_off = codes_base[ii] - (uint64_t)ops_base[in.job];
       w.traj = in.traj;
       w.frame_in = 2 * ii + (r & 1);
       w.frame_out = 2 * ii + ((r + 1) & 1);
@@ -280,7 +302,7 @@ int qoa_batch_run(qoa_batch* b) {
   QOA_HIP_CHECK(hipEventRecord(b->ev0, b->stream));
   for (const Round& rd : b->rounds) {
     for (const ItemGroup& g : rd.groups) {
-      launch_group(b->stream, g, b->states, rd.d_items, b->d_ops, b->d_pool, b->frames);
+      launch_group(b->stream, g, b->states, rd.d_items, b->d_ops, b->d_pool, b->frames, b->codes);
       QOA_HIP_CHECK(hipGetLastError());
     }
   }

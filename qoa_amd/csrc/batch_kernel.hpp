@@ -24,15 +24,15 @@ constexpr uint32_t batch_threads(uint32_t k) { return k <= 10 ? 64u : 1u << (k -
 
 struct WorkItem {
   uint64_t state_off;    // first amplitude of the instance's state
-  uint64_t seed;         // trajectory noise key
-  double p1, p2;
+  uint64_t codes_off;    // noise draws of op i of the op buffer: codes[codes_off + i]
+                         // (the sum taken mod 2^64)
   uint32_t ops_begin, nops;
   uint32_t block_begin;  // first block of this item in its round
   uint32_t traj;
   uint32_t frame_in, frame_out;  // slots of the X frame before / after the pass
   uint8_t k, init, noise, pad0;
   uint8_t spos[16];
-  uint8_t pad1[4];
+  uint8_t pad1[20];
 };
 static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
 
@@ -50,7 +50,8 @@ void k_batch_reg(double2* __restrict__ states,
                                                             int nitems,
                                                             const TileOp* __restrict__ ops,
                                                             const double* __restrict__ pool,
-                                                            uint64_t* __restrict__ frames) {
+                                                            uint64_t* __restrict__ frames,
+                                                            const uint32_t* __restrict__ codes) {
   extern __shared__ double2 lds[];
   int lo = 0, hi = nitems - 1;
   while (lo < hi) {  // last item whose first block is <= blockIdx.x (uniform)
@@ -74,10 +75,7 @@ void k_batch_reg(double2* __restrict__ states,
   const bool act = !SWAPS || threadIdx.x < (1u << (it.k - kRegBits));
   NoiseArgs nz;
   nz.on = it.noise;
-  nz.seed = it.seed;
-  nz.p1 = it.p1;
-  nz.p2 = it.p2;
-  nz.traj = it.traj;
+  nz.codes = codes + (it.codes_off + it.ops_begin);
   nz.frame = (it.noise && !it.init) ? frames[it.frame_in] : 0ULL;
   const uint64_t fout = reg_tile<I, false, NOISE, true, SWAPS>(states + it.state_off, base, geo, ops + it.ops_begin,
                                     (int)it.nops, pool, lds, act, it.init != 0, nz);
@@ -136,13 +134,38 @@ inline void batch_kernel_attrs() {
 // measured 125 jobs/s against 190 with it.
 inline void launch_group(hipStream_t st, const ItemGroup& g, double2* states,
                          const WorkItem* d_items, const TileOp* ops, const double* pool,
-                         uint64_t* frames) {
+                         uint64_t* frames, const uint32_t* codes) {
   const size_t lbytes = sizeof(double2) * reg_xchg_elems(g.kmax);
   auto fn = g.noise ? k_batch_reg<uint32_t, true, true>
                     : (g.kmax <= 10 ? k_batch_reg<uint32_t, false, true>
                                     : k_batch_reg<uint32_t, false, false>);
   hipLaunchKernelGGL(fn, dim3((unsigned)g.blocks), dim3(g.threads), lbytes, st, states,
-                     d_items + g.first, (int)g.count, ops, pool, frames);
+                     d_items + g.first, (int)g.count, ops, pool, frames, codes);
+}
+
+// Trajectory draws of one job: word t * nops + o holds, 2 bits per operand
+// j of op o, traj_pauli(seed, t, gi, qphys[j], p1 or p2) for trajectory t
+// (the counters of the in-kernel channel, drawn once instead of in every
+// tile).
+static __global__ __launch_bounds__(256) void k_draw_codes(const TileOp* __restrict__ ops, uint32_t nops,
+                                                    uint64_t seed, double p1, double p2,
+                                                    uint32_t ntraj, uint32_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (uint64_t)ntraj * nops) return;
+  const uint32_t t = (uint32_t)(i / nops), o = (uint32_t)(i % nops);
+  const TileOp& op = ops[o];
+  uint32_t w = 0;
+  for (uint32_t j = 0; j < op.nq && j < 3; ++j)
+    w |= traj_pauli(seed, t, op.gi, op.qphys[j], op.psel == 1 ? p1 : p2) << (2 * j);
+  out[i] = w;
+}
+
+inline void draw_codes(hipStream_t st, const TileOp* ops, uint32_t nops, uint64_t seed, double p1,
+                       double p2, uint32_t ntraj, uint32_t* out) {
+  const uint64_t n = (uint64_t)ntraj * nops;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_draw_codes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ops, nops,
+                     seed, p1, p2, ntraj, out);
 }
 
 }  // namespace tile

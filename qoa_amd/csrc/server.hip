@@ -103,6 +103,7 @@ struct SrvJob {
   uint64_t state_off = 0, state_bytes = 0;  // arena bytes
   uint64_t aux_off = 0, aux_bytes = 0;      // arena bytes: cs + samples + op pool
   uint64_t ops_off = 0, nops = 0;           // op slots in the op pool
+  uint64_t codes_word = 0;                  // noise draws (arena as 32-bit words)
   uint32_t frame_block = 0;
   uint32_t next_pass = 0;
   hipEvent_t done_ev = nullptr;
@@ -174,6 +175,10 @@ uint64_t job_aux_bytes(const SrvJob& j) {
   b += (uint64_t)j.desc.trajectories * j.samples_per_traj * 8;                // samples
   b = (b + 255) / 256 * 256;
   b += j.plan.pool.size() * 8 + 8;                                            // MAT4 pool
+  if (j.desc.p1 > 0.0 || j.desc.p2 > 0.0) {                                   // noise draws
+    b = (b + 255) / 256 * 256;
+    b += (uint64_t)j.desc.trajectories * j.plan.ops.size() * 4;
+  }
   return b;
 }
 
@@ -285,6 +290,14 @@ bool admit(qoa_server* s, const std::shared_ptr<SrvJob>& j) {
                          s->upload_stream);
   // host vectors are pageable: wait for the upload (not for running steps)
   (void)hipStreamSynchronize(s->upload_stream);
+  if (j->desc.p1 > 0.0 || j->desc.p2 > 0.0) {  // trajectory draws, before the job's first step
+    uint64_t codes_off = pool_off + j->plan.pool.size() * 8 + 8;
+    codes_off = (codes_off + 255) / 256 * 256;
+    j->codes_word = codes_off / 4;
+    tile::draw_codes(s->stream, s->d_ops + oo, (uint32_t)j->plan.ops.size(), j->desc.noise_seed,
+                     j->desc.p1, j->desc.p2, j->desc.trajectories,
+                     reinterpret_cast<uint32_t*>(s->arena_base + codes_off));
+  }
   j->t_admit = now_s(s);
   return true;
 }
@@ -308,9 +321,7 @@ int launch_step(qoa_server* s) {
       tile::WorkItem w;
       std::memset(&w, 0, sizeof(w));
       w.state_off = j->state_off / 16 + (uint64_t)t * len;
-      w.seed = j->desc.noise_seed;
-      w.p1 = j->desc.p1;
-      w.p2 = j->desc.p2;
+      w.codes_off = j->codes_word + (uint64_t)t * j->plan.ops.size() - j->ops_off;
       w.noise = (j->desc.p1 > 0.0 || j->desc.p2 > 0.0) ? 1 : 0;
       w.traj = t;
       w.frame_in = j->frame_block * kFrameBlock + 2 * t + (r & 1);
@@ -337,7 +348,8 @@ int launch_step(qoa_server* s) {
                                hipMemcpyHostToDevice, s->stream));
   for (const tile::ItemGroup& g : groups) {
     tile::launch_group(s->stream, g, (double2*)s->arena_base, s->d_items[slot], s->d_ops,
-                       (const double*)s->arena_base, s->d_frames);
+                       (const double*)s->arena_base, s->d_frames,
+                       reinterpret_cast<const uint32_t*>(s->arena_base));
     QOA_HIP_CHECK(hipGetLastError());
   }
   for (auto& j : last) {  // sample each trajectory and copy the samples back

@@ -1022,9 +1022,9 @@ __device__ __forceinline__ void flip_bits(double2 (&v)[kR], const RegLayout& L, 
 
 // Trajectory noise state of one tile pass (on == 0: no noise).
 struct NoiseArgs {
-  uint64_t seed;
-  double p1, p2;
-  uint32_t traj;
+  // the pass's trajectory draws: word o holds 2 bits per operand of op o
+  // (0 = no error, 1..3 = X, Y, Z), drawn by traj_pauli before the passes
+  const uint32_t* codes;
   uint32_t on;
   uint64_t frame;  // X frame entering the pass: logical index = physical ^ frame
 };
@@ -1347,11 +1347,11 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     }
     const uint32_t nq = op.w[2] >> 24;
     if (NOISE && nz.on && nq) {  // trajectory errors after the gate, operand order
-      const double p = (op.w[11] & 0xFFu) == 1 ? nz.p1 : nz.p2;
+      const uint32_t drawn = nz.codes[p - ops];
       for (uint32_t j = 0; j < nq; ++j) {
         const uint32_t q =
             j == 0 ? (op.w[11] >> 8) & 0xFFu : j == 1 ? (op.w[11] >> 16) & 0xFFu : op.w[11] >> 24;
-        const uint32_t code = traj_pauli(nz.seed, nz.traj, op.gi(), q, p);
+        const uint32_t code = (drawn >> (2 * j)) & 3u;
         if (!code) continue;
         int lb = -1;
         for (int s = 0; s < k; ++s)
