@@ -1299,15 +1299,14 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     }
   }
 
-  // Without noise the next op's dispatch words are loaded while the current
-  // op runs, and the current op's coefficients at the top of its iteration
-  // (they are read after dispatch): only the dispatch words pass from one
-  // iteration to the next. The noise handling needs those scalar registers,
-  // so with noise each op is loaded at the top of its iteration.
+  // The next op's dispatch words are loaded while the current op runs, and
+  // the current op's other words at the top of its iteration (they are read
+  // after dispatch): only the dispatch words pass from one iteration to the
+  // next.
   // The op after the last one is read too (its words are not used): every
   // op buffer ends with one spare op.
   OpRegs op;
-  if (!NOISE && nops > 0) {
+  if (nops > 0) {
     load_head(op, ops);
     // Consume the first op's words here: the wait for them then sits before
     // the loop.
@@ -1316,12 +1315,8 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
   const TileOp* const ops_end = ops + nops;
   for (const TileOp* p = ops; p != ops_end; ++p) {
     OpRegs next;
-    if (NOISE) {
-      op = load_op(p);
-    } else {
-      load_head(next, p + 1);
-      load_coefs(op, p);
-    }
+    load_head(next, p + 1);
+    load_coefs(op, p);
     if (op.kind() == TK_LAYOUT) {  // the new slot bits in ins
 #pragma unroll
       for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(op.w[1] >> (8 * j));
@@ -1374,11 +1369,9 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         }
       }
     }
-    if (!NOISE) {
 #pragma unroll
-      for (int i = 0; i < kOpWords; ++i)
-        if (head_word(i)) op.w[i] = next.w[i];
-    }
+    for (int i = 0; i < kOpWords; ++i)
+      if (head_word(i)) op.w[i] = next.w[i];
   }
   if (act) {  // store from the final layout
     gthr = 0;
