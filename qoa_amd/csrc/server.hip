@@ -321,8 +321,9 @@ int launch_step(qoa_server* s) {
       tile::WorkItem w;
       std::memset(&w, 0, sizeof(w));
       w.state_off = j->state_off / 16 + (uint64_t)t * len;
-      w.codes_off = j->codes_word + (uint64_t)t * j->plan.ops.size() - j->ops_off;
       w.noise = (j->desc.p1 > 0.0 || j->desc.p2 > 0.0) ? 1 : 0;
+      if (w.noise)  // the job's draw table in the arena (noise-free jobs have none)
+        w.codes_off = j->codes_word + (uint64_t)t * j->plan.ops.size() - j->ops_off;
       w.traj = t;
       w.frame_in = j->frame_block * kFrameBlock + 2 * t + (r & 1);
       w.frame_out = j->frame_block * kFrameBlock + 2 * t + ((r + 1) & 1);

@@ -1317,7 +1317,8 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
     OpRegs next;
     load_head(next, p + 1);
     load_coefs(op, p);
-    const uint32_t drawn = NOISE ? nz.codes[p - ops] : 0u;  // read after the op's work
+    // the op's draws (read after its work; items without noise have no table)
+    const uint32_t drawn = (NOISE && nz.on) ? nz.codes[p - ops] : 0u;
     if (op.kind() == TK_LAYOUT) {  // the new slot bits in ins
 #pragma unroll
       for (int j = 0; j < kRegBits; ++j) rpos[j] = (uint8_t)(op.w[1] >> (8 * j));
@@ -1342,7 +1343,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
       reg_op<F>(v, op, L, pool);
     }
     const uint32_t nq = op.w[2] >> 24;
-    if (NOISE && nz.on && nq) {  // trajectory errors after the gate, operand order
+    if (NOISE && drawn) {  // trajectory errors after the gate, operand order
       for (uint32_t j = 0; j < nq; ++j) {
         const uint32_t q =
             j == 0 ? (op.w[11] >> 8) & 0xFFu : j == 1 ? (op.w[11] >> 16) & 0xFFu : op.w[11] >> 24;

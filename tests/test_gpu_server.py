@@ -32,7 +32,8 @@ def _job(seed, n):
 
 def test_server_job_alone_equals_in_mix():
     """The same job (16 noisy trajectories) gives identical samples when it
-    runs alone and when it shares every step with 30 other jobs."""
+    runs alone and when it shares every step with 30 other jobs, noisy and
+    noise-free."""
     target_n, T = 11, 16
     g = _job(5, target_n)
     with Server(budget_bytes=2 << 30) as srv:
@@ -42,8 +43,9 @@ def test_server_job_alone_equals_in_mix():
         ids = []
         for k in range(30):
             n = 6 + (k % 9)
-            ids.append(srv.submit(n, _job(100 + k, n), trajectories=8, p1=1e-3, p2=1e-2,
-                                  noise_seed=k))
+            noisy = k % 3 != 0  # noise-free jobs share the noisy launches too
+            ids.append(srv.submit(n, _job(100 + k, n), trajectories=8, p1=1e-3 if noisy else 0.0,
+                                  p2=1e-2 if noisy else 0.0, noise_seed=k))
             if k == 12:
                 jid2 = srv.submit(target_n, g, trajectories=T, p1=0.02, p2=0.05, noise_seed=99)
         mixed = srv.wait(jid2)
