@@ -120,7 +120,8 @@ int qoa_state_upload(qoa_state* s, const double* amps, uint64_t n_amps);
 int qoa_state_download(qoa_state* s, double* amps, uint64_t n_amps);
 int qoa_state_synchronize(qoa_state* s);
 /* Copies the amplitudes of src into dst (same n); c128 -> c64 and c64 ->
-   c128 conversions are allowed. Runs on dst's stream after src's work. */
+   c128 conversions are allowed. Starts after src's work and returns when the
+   copy is complete, so work may follow at once on either handle. */
 int qoa_state_copy(qoa_state* dst, const qoa_state* src);
 
 /* ---- layer 1: src/vectorization.rs free functions ---------------------- */
@@ -203,6 +204,19 @@ int qoa_apply_pauli_codes(qoa_state* s, const uint8_t* codes, uint32_t n_codes);
 int qoa_run_gates(qoa_state* s, const qoa_gate* gates, uint32_t n_gates,
                   const double* matrices, uint32_t flags);
 
+/* One noisy trajectory as a plain per-gate loop: each gate's sweep, then
+   for each operand qubit an X, Y or Z sweep with probability p1 (one-qubit
+   gate) or p2 (two- and three-qubit gates), drawn by Philox4x32-10 with
+   counter (gate index, qubit, traj, 0x7A3C5E1F) and key seed, the draws the
+   batched and serving kernels make in-kernel (the serving workload's
+   operand channel; the reference's own channel is qoa_qs_apply_noise,
+   quantum_state.rs:872-885). order (NULL: 0..n_gates-1) is the execution
+   order; the draw counter is always a gate's index in `gates`. Operands must
+   be distinct and in range; MEASURE/RESET are rejected. */
+int qoa_run_trajectory(qoa_state* s, const qoa_gate* gates, uint32_t n_gates,
+                       const double* matrices, uint32_t flags, uint64_t seed, uint32_t traj,
+                       double p1, double p2, const uint32_t* order);
+
 /* Fused execution: the gate list is planned into passes; each pass loads
    2^tile_qubits-amplitude tiles (the low `low_qubits` qubits plus qubits
    chosen per pass) into LDS once, applies every gate of the pass there and
@@ -224,8 +238,9 @@ int qoa_plan_fused(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
 
 /* Introspection of a fused plan (host only): copies the op program (records
    of qoa_tile_op_size() bytes) and, per pass, the tile qubit positions
-   (16 bytes), the op range and the kernel mode. Returns the counts; pass NULL
-   buffers to query sizes. */
+   (16 bytes), and per pass three words: op begin, op end, kernel mode.
+   Returns the counts; pass NULL buffers to query sizes; buffers too small
+   for the plan give QOA_ERR_ARG. */
 int qoa_tile_op_size(void);
 int qoa_plan_program(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
                      const double* matrices, const qoa_fuse_opts* opts, void* ops_out,

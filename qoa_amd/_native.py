@@ -115,6 +115,7 @@ _SIGS = {
     "qoa_probabilities": ([P, P, U64], I32),
     "qoa_apply_pauli_codes": ([P, P, U32], I32),
     "qoa_run_gates": ([P, P, U32, P, U32], I32),
+    "qoa_run_trajectory": ([P, P, U32, P, U32, U64, U32, D, D, P], I32),
     "qoa_run_fused": ([P, P, U32, P, P], I32),
     "qoa_plan_fused": ([U32, P, U32, P, P, C.POINTER(U32)], I32),
     "qoa_tile_op_size": ([], I32),

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "qoa_internal.hpp"
+#include "traj_noise.hpp"
 
 namespace qoa {
 
@@ -403,9 +404,14 @@ int qoa_state_copy(qoa_state* dst, const qoa_state* src) {
   if (dst->precision == src->precision) {
     const size_t bytes = dst->len * (dst->precision == QOA_C128 ? 16 : 8);
     QOA_HIP_CHECK(hipMemcpyAsync(dst->amps, src->amps, bytes, hipMemcpyDeviceToDevice, dst->stream));
-    return QOA_OK;
+  } else {
+    const int rc = launch_convert(dst, src);
+    if (rc) return rc;
   }
-  return launch_convert(dst, src);
+  // The copy reads src on dst's stream: wait for it, so that work enqueued
+  // next on src's own stream cannot overwrite src while it is being read.
+  QOA_HIP_CHECK(hipStreamSynchronize(dst->stream));
+  return QOA_OK;
 }
 
 // ---- layer 1 ------------------------------------------------------------------------
@@ -646,6 +652,47 @@ int qoa_run_gates(qoa_state* s, const qoa_gate* gates, uint32_t n_gates, const d
   for (uint32_t k = 0; k < n_gates; ++k) {
     int rc = apply_gate(s, &gates[k], mats, flags);
     if (rc) return rc;
+  }
+  return QOA_OK;
+}
+
+int qoa_run_trajectory(qoa_state* s, const qoa_gate* gates, uint32_t n_gates,
+                       const double* mats, uint32_t flags, uint64_t seed, uint32_t traj,
+                       double p1, double p2, const uint32_t* order) {
+  CHECK_STATE(s); CHECK_C128(s);
+  if (!gates && n_gates) return set_error(QOA_ERR_ARG, "gates is NULL");
+  if (!(p1 >= 0.0 && p1 <= 1.0 && p2 >= 0.0 && p2 <= 1.0))
+    return set_error(QOA_ERR_ARG, "noise probabilities outside [0,1]");
+  for (uint32_t k = 0; k < n_gates; ++k) {  // validate before the first launch
+    const uint32_t gi = order ? order[k] : k;
+    if (gi >= n_gates) return set_error(QOA_ERR_ARG, "order[%u] = %u out of range", k, gi);
+    const qoa_gate& g = gates[gi];
+    const uint32_t ar = traj_arity(g.op);
+    const uint32_t q[3] = {g.q0, g.q1, g.q2};
+    for (uint32_t j = 0; j < ar; ++j) {
+      if (q[j] >= s->n) return set_error(QOA_ERR_ARG, "gate %u: operand %u out of range", gi, q[j]);
+      for (uint32_t i = 0; i < j; ++i)
+        if (q[i] == q[j]) return set_error(QOA_ERR_ARG, "gate %u: repeated operand", gi);
+    }
+    if (g.op == QOA_OP_MEASURE || g.op == QOA_OP_RESET || g.op == QOA_OP_RESET_ALL)
+      return set_error(QOA_ERR_ARG, "gate %u: op 0x%02x has no trajectory form", gi, g.op);
+  }
+  for (uint32_t k = 0; k < n_gates; ++k) {
+    const uint32_t gi = order ? order[k] : k;
+    const qoa_gate& g = gates[gi];
+    int rc = apply_gate(s, &g, mats, flags);
+    if (rc) return rc;
+    const uint32_t ar = traj_arity(g.op);
+    const double p = ar == 1 ? p1 : p2;
+    if (!(p > 0.0)) continue;
+    const uint32_t q[3] = {g.q0, g.q1, g.q2};
+    for (uint32_t j = 0; j < ar; ++j) {  // errors after the gate, operand order
+      const uint32_t code = traj_pauli(seed, traj, gi, q[j], p);
+      if (code == 1) rc = launch_pauli_x(s, q[j], 0);
+      else if (code == 2) rc = launch_pauli_y(s, q[j], 0);
+      else if (code == 3) rc = launch_diag_mask(s, 1ULL << q[j], true, Cx{0.0, 0.0});
+      if (rc) return rc;
+    }
   }
   return QOA_OK;
 }

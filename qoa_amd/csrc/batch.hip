@@ -218,13 +218,15 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
                 hipMemcpyHostToDevice) != hipSuccess)
     rc = set_error(QOA_ERR_HIP, "pool upload failed");
   // noise draws: instance ii's words start at codes_base[ii] (a job's
-  // trajectories are consecutive instances, so its table is t * nops + o)
+  // trajectories are consecutive instances, so its table is t * nops + o);
+  // only noisy jobs have words (the kernel reads none for the others)
   std::vector<uint64_t> codes_base(b->inst.size(), 0);
   if (rc == QOA_OK && b->noisy) {
     uint64_t words = 0;
     for (size_t ii = 0; ii < b->inst.size(); ++ii) {
+      const qoa_job_desc& jd = b->jobs[b->inst[ii].job];
       codes_base[ii] = words;
-      words += b->plans[b->inst[ii].job].ops.size();
+      if (jd.p1 > 0.0 || jd.p2 > 0.0) words += b->plans[b->inst[ii].job].ops.size();
     }
     if (hipMalloc(&b->codes, sizeof(uint32_t) * (words + 1)) != hipSuccess) {
       (void)hipGetLastError();

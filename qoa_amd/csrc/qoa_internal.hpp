@@ -128,6 +128,12 @@ int launch_convert(qoa_state* dst, const qoa_state* src);
 int launch_upload_c64(qoa_state* s, const double* dev_src);
 // sample.hip: `count` samples from each of `ntraj` n-qubit states `stride`
 // amplitudes apart; results XOR the state's X frame (frames may be NULL).
+// cs: sample_scratch_doubles(n, ntraj) doubles of device scratch (per
+// trajectory the 2^n / min(2^n, 4096) chunk running sums and two slots).
+inline uint64_t sample_scratch_doubles(uint32_t n, uint32_t ntraj) {
+  const uint64_t len = 1ULL << n;
+  return (uint64_t)ntraj * (len / (len < 4096 ? len : 4096) + 2);
+}
 int launch_sample(hipStream_t st, const double2* states, uint64_t stride, uint32_t n,
                   uint32_t ntraj, uint64_t seed, uint32_t count, const uint64_t* frames,
                   uint32_t frame_stride, double* cs, uint64_t* out);

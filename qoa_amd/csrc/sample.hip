@@ -46,7 +46,8 @@ __device__ __forceinline__ double sample_uniform(uint64_t seed, uint32_t traj, u
 // One block per (chunk, trajectory): chunk sums.
 __global__ __launch_bounds__(kSampThreads) void k_chunk_sums(const double2* __restrict__ states,
                                                              uint64_t stride, uint32_t C,
-                                                             uint32_t nch, double* __restrict__ cs) {
+                                                             uint32_t nch, uint64_t cs_stride,
+                                                             double* __restrict__ cs) {
   __shared__ double part[kSampThreads];
   const uint32_t c = blockIdx.x, t = blockIdx.y;
   const uint32_t runs = C < kSampThreads ? C : kSampThreads, per = C / runs;
@@ -60,52 +61,87 @@ __global__ __launch_bounds__(kSampThreads) void k_chunk_sums(const double2* __re
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (uint32_t r = 0; r < runs; ++r) s += part[r];
-    cs[(uint64_t)t * nch + c] = s;
+    cs[(uint64_t)t * cs_stride + c] = s;
   }
 }
 
-// One block per trajectory: `count` samples; out[t*count + s] = index ^ frame.
+// One block per trajectory: replaces the chunk sums by their running sums
+// (acc += cs[c], left to right: the same operations, in the same order, as
+// the walk over chunks, so prefix[c] is exactly the walk's acc + cs[c]), and
+// stores after them the last chunk with a non-zero sum and the running sum
+// before it (the walk's fallback when rounding leaves no chunk above the
+// target). Chunk sums pass through LDS in tiles; one thread adds.
+constexpr uint32_t kPrefixTile = 2048;
+__global__ __launch_bounds__(kSampThreads) void k_prefix(double* __restrict__ cs, uint32_t nch,
+                                                         uint64_t cs_stride) {
+  __shared__ double tile[kPrefixTile];
+  double* c = cs + (uint64_t)blockIdx.x * cs_stride;
+  double acc = 0.0, acc_last = 0.0;
+  int64_t lastnz = -1;
+  for (uint32_t base = 0; base < nch; base += kPrefixTile) {
+    const uint32_t m = nch - base < kPrefixTile ? nch - base : kPrefixTile;
+    for (uint32_t i = threadIdx.x; i < m; i += kSampThreads) tile[i] = c[base + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t i = 0; i < m; ++i) {
+        const double v = tile[i];
+        if (v > 0.0) {
+          lastnz = (int64_t)(base + i);
+          acc_last = acc;
+        }
+        acc += v;
+        tile[i] = acc;
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < m; i += kSampThreads) c[base + i] = tile[i];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    c[nch] = (double)lastnz;
+    c[nch + 1] = acc_last;
+  }
+}
+
+// Grid (trajectory, sample group): samples s = blockIdx.y, + gridDim.y, ...
+// of trajectory blockIdx.x; out[t*count + s] = index ^ frame. The chunk is
+// the first one whose running sum exceeds u * total (a binary search: the
+// running sums never decrease); inside it the runs and then the amplitudes
+// are walked left to right.
 __global__ __launch_bounds__(kSampThreads) void k_walk(const double2* __restrict__ states,
                                                        uint64_t stride, uint32_t C, uint32_t nch,
-                                                       const double* __restrict__ cs,
-                                                       uint64_t seed, uint32_t count,
+                                                       const double* __restrict__ pre_all,
+                                                       uint64_t cs_stride, uint64_t seed,
+                                                       uint32_t count,
                                                        const uint64_t* __restrict__ frames,
                                                        uint32_t frame_stride,
                                                        uint64_t* __restrict__ out) {
   __shared__ double part[kSampThreads];
-  __shared__ uint32_t sel_chunk, sel_run;
+  __shared__ uint32_t sel_chunk;
   __shared__ double acc_s;
   const uint32_t t = blockIdx.x;
   const uint32_t runs = C < kSampThreads ? C : kSampThreads, per = C / runs;
-  const double* cst = cs + (uint64_t)t * nch;
+  const double* pre = pre_all + (uint64_t)t * cs_stride;
   const double2* a = states + t * stride;
   const uint64_t frame = frames ? frames[(uint64_t)t * frame_stride] : 0ULL;
-  double total = 0.0;
-  if (threadIdx.x == 0)
-    for (uint32_t c = 0; c < nch; ++c) total += cst[c];
-  for (uint32_t s = 0; s < count; ++s) {
+  const double total = pre[nch - 1];
+  for (uint32_t s = blockIdx.y; s < count; s += gridDim.y) {
+    const double target = sample_uniform(seed, t, s) * total;
     if (threadIdx.x == 0) {
-      const double target = sample_uniform(seed, t, s) * total;
-      double acc = 0.0;
-      int chosen = -1, lastnz = -1;
-      double acc_last = 0.0;
-      for (uint32_t c = 0; c < nch; ++c) {
-        if (cst[c] > 0.0) {
-          lastnz = (int)c;
-          acc_last = acc;
-        }
-        if (acc + cst[c] > target) {
-          chosen = (int)c;
-          break;
-        }
-        acc += cst[c];
+      uint32_t lo = 0, hi = nch;  // first c with pre[c] > target
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (pre[mid] > target) hi = mid;
+        else lo = mid + 1;
       }
-      if (chosen < 0) {
-        chosen = lastnz < 0 ? 0 : lastnz;
-        acc = lastnz < 0 ? 0.0 : acc_last;
+      if (lo < nch) {
+        sel_chunk = lo;
+        acc_s = lo ? pre[lo - 1] : 0.0;
+      } else {
+        const int64_t lastnz = (int64_t)pre[nch];
+        sel_chunk = lastnz < 0 ? 0u : (uint32_t)lastnz;
+        acc_s = lastnz < 0 ? 0.0 : pre[nch + 1];
       }
-      sel_chunk = (uint32_t)chosen;
-      acc_s = acc;
     }
     __syncthreads();
     const double2* ch = a + (uint64_t)sel_chunk * C;
@@ -116,7 +152,6 @@ __global__ __launch_bounds__(kSampThreads) void k_walk(const double2* __restrict
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      const double target = sample_uniform(seed, t, s) * total;
       double acc = acc_s;
       int chosen = -1, lastnz = -1;
       double acc_last = acc;
@@ -157,17 +192,21 @@ __global__ __launch_bounds__(kSampThreads) void k_walk(const double2* __restrict
 }  // namespace
 
 // Samples `count` outcomes from each of `ntraj` states of n qubits laid out
-// `stride` amplitudes apart. `cs` is device scratch of ntraj * nch doubles.
+// `stride` amplitudes apart. `cs` is device scratch of
+// sample_scratch_doubles(n, ntraj) doubles.
 int launch_sample(hipStream_t st, const double2* states, uint64_t stride, uint32_t n,
                   uint32_t ntraj, uint64_t seed, uint32_t count, const uint64_t* frames,
                   uint32_t frame_stride, double* cs, uint64_t* out) {
   const uint64_t len = 1ULL << n;
   const uint32_t C = (uint32_t)(len < 4096 ? len : 4096);
   const uint32_t nch = (uint32_t)(len / C);
+  const uint64_t cs_stride = (uint64_t)nch + 2;
   hipLaunchKernelGGL(k_chunk_sums, dim3(nch, ntraj), dim3(kSampThreads), 0, st, states, stride,
-                     C, nch, cs);
-  hipLaunchKernelGGL(k_walk, dim3(ntraj), dim3(kSampThreads), 0, st, states, stride, C, nch, cs,
-                     seed, count, frames, frame_stride, out);
+                     C, nch, cs_stride, cs);
+  hipLaunchKernelGGL(k_prefix, dim3(ntraj), dim3(kSampThreads), 0, st, cs, nch, cs_stride);
+  const uint32_t groups = count < 1024u ? count : 1024u;
+  hipLaunchKernelGGL(k_walk, dim3(ntraj, groups), dim3(kSampThreads), 0, st, states, stride, C,
+                     nch, cs, cs_stride, seed, count, frames, frame_stride, out);
   QOA_HIP_CHECK(hipGetLastError());
   return QOA_OK;
 }
@@ -180,11 +219,9 @@ extern "C" int qoa_sample(qoa_state* s, uint64_t seed, uint32_t count, uint64_t*
   if (s->precision != QOA_C128) return set_error(QOA_ERR_UNSUPPORTED, "sampling needs QOA_C128");
   if (!out && count) return set_error(QOA_ERR_ARG, "out is NULL");
   if (count == 0) return QOA_OK;
-  const uint32_t C = (uint32_t)(s->len < 4096 ? s->len : 4096);
-  const uint64_t nch = s->len / C;
   double* cs = nullptr;
   uint64_t* dout = nullptr;
-  if (hipMalloc(&cs, sizeof(double) * nch) != hipSuccess ||
+  if (hipMalloc(&cs, sizeof(double) * sample_scratch_doubles(s->n, 1)) != hipSuccess ||
       hipMalloc(&dout, sizeof(uint64_t) * count) != hipSuccess) {
     (void)hipGetLastError();
     if (cs) hipFree(cs);

@@ -168,9 +168,7 @@ uint64_t job_state_bytes(const qoa_job_desc& d) {
 }
 
 uint64_t job_aux_bytes(const SrvJob& j) {
-  const uint64_t len = 1ULL << j.desc.n_qubits;
-  const uint64_t nch = len / (len < 4096 ? len : 4096);
-  uint64_t b = (uint64_t)j.desc.trajectories * nch * 8;                       // chunk sums
+  uint64_t b = sample_scratch_doubles(j.desc.n_qubits, j.desc.trajectories) * 8;  // chunk sums
   b = (b + 255) / 256 * 256;
   b += (uint64_t)j.desc.trajectories * j.samples_per_traj * 8;                // samples
   b = (b + 255) / 256 * 256;
@@ -259,9 +257,7 @@ bool admit(qoa_server* s, const std::shared_ptr<SrvJob>& j) {
   s->free_pinned.pop_back();
   j->pinned = s->pinned_pool + (uint64_t)j->pinned_slot * kMaxTraj;
   // op program (with MAT4 offsets into the job's own pool region)
-  const uint64_t len = 1ULL << j->desc.n_qubits;
-  const uint64_t nch = len / (len < 4096 ? len : 4096);
-  uint64_t pool_off = ao + (uint64_t)j->desc.trajectories * nch * 8;
+  uint64_t pool_off = ao + sample_scratch_doubles(j->desc.n_qubits, j->desc.trajectories) * 8;
   pool_off = (pool_off + 255) / 256 * 256;
   pool_off += (uint64_t)j->desc.trajectories * j->samples_per_traj * 8;
   pool_off = (pool_off + 255) / 256 * 256;
@@ -297,6 +293,10 @@ bool admit(qoa_server* s, const std::shared_ptr<SrvJob>& j) {
     tile::draw_codes(s->stream, s->d_ops + oo, (uint32_t)j->plan.ops.size(), j->desc.noise_seed,
                      j->desc.p1, j->desc.p2, j->desc.trajectories,
                      reinterpret_cast<uint32_t*>(s->arena_base + codes_off));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {  // the engine stops; waiters see the fatal flag
+      s->fatal = set_error(QOA_ERR_HIP, "noise draw launch failed: %s", hipGetErrorString(e));
+    }
   }
   j->t_admit = now_s(s);
   return true;
@@ -356,9 +356,8 @@ int launch_step(qoa_server* s) {
   for (auto& j : last) {  // sample each trajectory and copy the samples back
     const uint32_t T = j->desc.trajectories, P = (uint32_t)j->plan.passes.size();
     const uint64_t len = 1ULL << j->desc.n_qubits;
-    const uint64_t nch = len / (len < 4096 ? len : 4096);
     double* cs = (double*)(s->arena_base + j->aux_off);
-    uint64_t so = j->aux_off + (uint64_t)T * nch * 8;
+    uint64_t so = j->aux_off + sample_scratch_doubles(j->desc.n_qubits, T) * 8;
     so = (so + 255) / 256 * 256;
     uint64_t* dsamp = (uint64_t*)(s->arena_base + so);
     const bool noisy = j->desc.p1 > 0.0 || j->desc.p2 > 0.0;
@@ -416,8 +415,8 @@ void engine_main(qoa_server* s) {
         break;  // arrival order: later jobs wait behind the head
       }
     }
-    int rc = QOA_OK;
-    if (!s->fatal) rc = launch_step(s);
+    // after a fatal error every admitted job finishes with that error
+    const int rc = s->fatal ? s->fatal : launch_step(s);
     if (rc) {
       s->fatal = rc;
       for (auto& j : s->active) finish_job(s, j, rc);

@@ -8,6 +8,7 @@
 #pragma once
 #include "planner.hpp"
 #include "qoa_internal.hpp"
+#include "traj_noise.hpp"
 
 #include <type_traits>
 
@@ -1029,31 +1030,7 @@ struct NoiseArgs {
   uint64_t frame;  // X frame entering the pass: logical index = physical ^ frame
 };
 
-// Philox4x32-10 (Salmon et al., SC'11) with counter (gi, q, traj, const) and
-// key = seed: 0 = no error, else 1 + floor(3 * o2 / 2^32) picks X, Y, Z.
-__device__ __forceinline__ uint32_t traj_pauli(uint64_t seed, uint32_t traj, uint32_t gi,
-                                               uint32_t q, double p) {
-  uint32_t c0 = gi, c1 = q, c2 = traj, c3 = 0x7A3C5E1Fu;
-  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    if (r > 0) {
-      k0 += 0x9E3779B9u;
-      k1 += 0xBB67AE85u;
-    }
-    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
-    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
-    c1 = (uint32_t)p1;
-    c3 = (uint32_t)p0;
-    c0 = n0;
-    c2 = n2;
-  }
-  const double u = (double)((((uint64_t)c1 << 32) | c0) >> 11) * 0x1.0p-53;
-  if (!(u < p)) return 0u;
-  return 1u + (uint32_t)(((uint64_t)c2 * 3ULL) >> 32);
-}
+// traj_pauli (traj_noise.hpp) draws the trajectory errors.
 
 template <int J, bool Y>
 __device__ __forceinline__ void reg_flip(double2 (&v)[kR]) {
@@ -1100,10 +1077,11 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
 // A TileOp as its 28 32-bit words, held in scalar registers: uniform loads of
 // whole dwords (a byte read would become a vector load and a wait), issued
 // one op ahead so the op loop never waits on them.
-// w0 = kind | t << 8 | t2 << 16 | nins << 24, w1 = ins[0..3], w3 = aux,
-// w4..w5 = gmask, w6..w7 = gval, w8..w23 = c[0..7], w24 = gi,
-// w25 = nq | psel << 8 | qphys[0] << 16 | qphys[1] << 24,
-// w26 = qphys[2] | rsel << 8 | rt << 16 | outside << 24, w27 = tl.
+// w0 = kind | t << 8 | t2 << 16 | nins << 24, w1 = ins[0..3],
+// w2 = rsel | rt << 8 | outside << 16 | nq << 24, w3 = tl,
+// w4..w5 = gmask, w6..w7 = gval, w8 = aux, w9 = lmask, w10 = gi,
+// w11 = psel | qphys[0] << 8 | qphys[1] << 16 | qphys[2] << 24,
+// w12..w27 = c[0..7] (the offsets below fix this layout).
 constexpr int kOpWords = 28;
 static_assert(sizeof(TileOp) == 4 * kOpWords && offsetof(TileOp, ins) == 4 &&
                   offsetof(TileOp, rsel) == 8 && offsetof(TileOp, rt) == 9 &&

@@ -177,6 +177,17 @@ class StateVector:
         check(self._lib.qoa_run_gates(self._h, ptr(g), len(g), None if m is None else ptr(m),
                                       flags))
 
+    def run_trajectory(self, gates, seed: int, traj: int, p1: float, p2: float, flags: int = 0,
+                       matrices: Optional[np.ndarray] = None, order=None) -> None:
+        """One noisy trajectory as a plain per-gate loop (each gate's sweep,
+        then the drawn Pauli sweeps on its operands)."""
+        g = gates_array(gates)
+        m = None if matrices is None else np.ascontiguousarray(matrices, dtype=np.complex128)
+        o = None if order is None else np.ascontiguousarray(order, dtype=np.uint32)
+        check(self._lib.qoa_run_trajectory(self._h, ptr(g), len(g), None if m is None else ptr(m),
+                                           flags, seed, traj, p1, p2,
+                                           None if o is None else ptr(o)))
+
     def run_fused(self, gates, tile_qubits: int = 0, low_qubits: int = 0, flags: int = 0,
                   matrices: Optional[np.ndarray] = None) -> None:
         g = gates_array(gates)
