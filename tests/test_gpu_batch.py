@@ -81,3 +81,42 @@ def test_noisy_trajectories_match_oracle(tile):
                 assert np.array_equal(out, ref), (j, n, t)
                 outs.append(out)
             assert any(not np.array_equal(outs[0], o) for o in outs[1:]), "noise had no effect"
+
+
+def shared_table_jobs(sizes, count, seed0):
+    """mixed_circuit jobs (every gate kind, U2/U4 included) with in-range
+    operands, their matrices moved into one shared table."""
+    jobs, tables, slot = [], [], 0
+    for j, n in enumerate(sizes):
+        g, mats = mixed_circuit(n, count, seed=seed0 + j)
+        g = g[g["q0"] < n].copy()  # trajectories need in-range operands
+        nslots = mats.size // 16
+        u2 = g["op"] == 0xE2
+        u4 = g["op"] == 0xE3
+        g["aux"][u2] += 4 * slot
+        g["aux"][u4] += slot
+        tables.append(mats)
+        slot += nslots
+        jobs.append((n, g))
+    return jobs, np.ascontiguousarray(np.concatenate(tables))
+
+
+@pytest.mark.parametrize("tile", [8, 10])
+def test_noisy_mixed_surface_matches_oracle(tile):
+    """Noisy trajectories of circuits with every gate kind (Toffoli/CSWAP
+    with three operands, X-type ops with controls outside the tile under the
+    X frame, H, CY, U2/U4): each trajectory equals the oracle trajectory in
+    planned order bit for bit, for jobs that take several passes."""
+    p1, p2, T = 0.05, 0.2, 3
+    jobs, table = shared_table_jobs([12, 14, 16], 160, seed0=40 + tile)
+    seeds = [7000 + j for j in range(len(jobs))]
+    with Batch(jobs, tile_qubits=tile, trajectories=T, matrices=table, p1=p1, p2=p2,
+               noise_seed=seeds) as b:
+        b.run()
+        b.synchronize()
+        for j, (n, g) in enumerate(jobs):
+            order = b.order(j)
+            for t in range(T):
+                ref = po.state(n)
+                po.run_trajectory(ref, g, seeds[j], t, p1, p2, matrices=table, order=order)
+                assert np.array_equal(b.download(j, t), ref), (tile, j, n, t)

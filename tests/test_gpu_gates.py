@@ -7,6 +7,7 @@ import pytest
 import pyoracle as po
 from kat_util import arr, assert_kat, kats, random_state
 from qoa_amd import StateVector
+from qoa_amd._native import GATE_DTYPE
 from test_oracle_kats import run_l1_oracle
 
 pytestmark = pytest.mark.gpu
@@ -154,7 +155,91 @@ def test_pauli_codes_fused_sweep(n):
             assert np.array_equal(sv.download(), ref), codes
 
 
-def test_large_target_31_qubit_index_path():
-    """n = 33 uses 64-bit indexing; exercised on a 2-qubit projection: only the
-    index arithmetic matters, checked on a sparse state via reductions."""
-    pytest.skip("33-qubit state needs 128 GiB; covered by the 64-bit template on n<=31 inputs")
+def test_copy_is_complete_before_the_source_changes():
+    """qoa_state_copy reads the source on the destination's stream; work
+    enqueued right after it on the source's own stream must not change what
+    the copy sees (2 GiB states, so the copy takes long enough to overlap)."""
+    from qoa_amd import workloads as W
+
+    n = 27
+    with StateVector(n) as a, StateVector(n) as b, StateVector(n) as c:
+        a.run_gates(W.layered_random(n, 1, seed=3))
+        c.copy_from(a)
+        b.copy_from(a)
+        a.apply_reset_all()  # overwrites the source at once
+        d, f = b.compare(c)
+        assert d == 0.0 and abs(f - 1.0) < 1e-12
+
+
+def test_32_qubit_state_index_path():
+    """A 32-qubit state (64 GiB): gates on qubits 30/31 and masks with bits
+    >= 31 run the 64-bit index kernels at real size. Checked by device
+    reductions only (norm, measure p0/p1, samples), against the same gates
+    on a 3-qubit oracle state where qubits (0, 30, 31) become (0, 1, 2), and
+    the fused run against the per-gate run on a second 64 GiB state."""
+    from qoa_amd._native import QOA_ERR_OOM, QoaError
+    from qoa_amd.server import sample_state
+
+    try:
+        sv = StateVector(32)
+    except QoaError as e:
+        if e.code == QOA_ERR_OOM:
+            pytest.skip("device has less than 64 GiB free for a 32-qubit state")
+        raise
+    m = {0: 0, 30: 1, 31: 2}
+    ref = po.state(3)
+    b = lambda q: 1 << q  # noqa: E731
+    with sv:
+        steps = [
+            ("hadamard", (31,)), ("cnot", (31, 0)), ("hadamard", (30,)), ("rx", (30, 0.9)),
+            ("phaseshift", (31, 1.3)), ("cz", (b(31), b(30))), ("cphase", (b(31), b(0), 0.7)),
+            ("ry", (0, 2.2)), ("swap", (31, 0)), ("hadamard", (31,)), ("rz", (30, -0.4)),
+        ]
+        for fn, args in steps:
+            if fn == "hadamard":
+                sv.apply_hadamard(args[0])
+                po.orc.hadamard(ref, 1 / math.sqrt(2), 0.0, m[args[0]])
+            elif fn in ("rx", "ry", "rz", "phaseshift"):
+                getattr(sv, "apply_" + fn)(args[0], args[1])
+                getattr(po.orc, fn)(ref, m[args[0]], args[1])
+            elif fn == "cnot":
+                sv.apply_cnot(*args)
+                po.orc.cnot(ref, m[args[0]], m[args[1]])
+            elif fn == "swap":
+                sv.apply_swap(*args)
+                po.orc.swap(ref, m[args[0]], m[args[1]])
+            elif fn == "cz":
+                sv.apply_cz(*args)
+                po.orc.cz(ref, 1 << m[args[0].bit_length() - 1], 1 << m[args[1].bit_length() - 1])
+            elif fn == "cphase":
+                sv.apply_controlled_phase_rotation(*args)
+                po.orc.cphase(ref, 1 << m[args[0].bit_length() - 1],
+                              1 << m[args[1].bit_length() - 1], args[2])
+        assert abs(sv.norm_sqr() - 1.0) < 1e-13
+        pr = np.abs(ref) ** 2
+        samples = sample_state(sv, seed=5, count=4000).astype(np.int64)
+        keep = (1 << 0) | (1 << 30) | (1 << 31)
+        assert np.all((samples & ~keep) == 0)
+        small = (samples & 1) | (((samples >> 30) & 1) << 1) | (((samples >> 31) & 1) << 2)
+        counts = np.bincount(small, minlength=8)
+        for i in range(8):
+            e = 4000 * pr[i]
+            assert abs(counts[i] - e) <= 5 * math.sqrt(e * (1 - pr[i])) + 1, (i, counts[i], e)
+        want_p0 = float(pr[[0, 1, 2, 3]].sum())  # qubit 31 (oracle bit 2) = 0
+        assert 0.05 < want_p0 < 0.95
+        o, p0, p1 = sv.measure_with(31, 0.999)  # r >= p0: outcome 1
+        assert o == 1 and abs(p0 - want_p0) < 1e-13 and abs(p1 - (1 - want_p0)) < 1e-13
+        sv.normalize()
+        post = sample_state(sv, seed=6, count=500).astype(np.int64)
+        assert np.all(((post >> 31) & 1) == 1) and np.all((post & ~keep) == 0)
+        # fused (64-bit tile indexing) equals per-gate on a second 64 GiB state
+        g = np.array([(op, q0, q1, 0, th, 0) for op, q0, q1, th in [
+            (0x05, 31, 0, 0.0), (0x17, 31, 2, 0.0), (0x22, 30, 0, 0.4), (0x35, 29, 31, 0.3),
+            (0x23, 1, 0, 1.1), (0x17, 0, 30, 0.0), (0x1E, 31, 1, 0.0), (0x05, 2, 0, 0.0),
+            (0x0F, 31, 0, 0.8), (0x0B, 30, 3, 0.0)]], dtype=GATE_DTYPE)
+        with StateVector(32) as other:
+            other.copy_from(sv)
+            sv.run_fused(g)
+            other.run_gates(g)
+            d, f = sv.compare(other)
+        assert d <= 1e-12 and f >= 1 - 1e-12, (d, f)

@@ -36,10 +36,14 @@ def test_server_job_alone_equals_in_mix():
     noise-free."""
     target_n, T = 11, 16
     g = _job(5, target_n)
+    clean_n = 10  # a noise-free job: it runs inside the noisy launches in the mix
+    gc = _job(6, clean_n)
     with Server(budget_bytes=2 << 30) as srv:
         jid = srv.submit(target_n, g, trajectories=T, p1=0.02, p2=0.05, noise_seed=99)
         alone = srv.wait(jid)
         assert alone.status == 0 and alone.samples.size == T
+        clean_alone = srv.wait(srv.submit(clean_n, gc, trajectories=8, noise_seed=123))
+        assert clean_alone.status == 0 and clean_alone.samples.size == 8
         ids = []
         for k in range(30):
             n = 6 + (k % 9)
@@ -48,12 +52,15 @@ def test_server_job_alone_equals_in_mix():
                                   p2=1e-2 if noisy else 0.0, noise_seed=k))
             if k == 12:
                 jid2 = srv.submit(target_n, g, trajectories=T, p1=0.02, p2=0.05, noise_seed=99)
+                jc2 = srv.submit(clean_n, gc, trajectories=8, noise_seed=123)
         mixed = srv.wait(jid2)
+        clean_mixed = srv.wait(jc2)
         for i in ids:
             assert srv.wait(i).status == 0
         st = srv.stats()
     assert np.array_equal(alone.samples, mixed.samples)
-    assert st["completed"] == 32 and st["fatal"] == 0
+    assert np.array_equal(clean_alone.samples, clean_mixed.samples)
+    assert st["completed"] == 34 and st["fatal"] == 0
 
 
 def test_server_samples_match_oracle_trajectories():
