@@ -813,9 +813,12 @@ extern "C" int qoa_plan_program(uint32_t n, const qoa_gate* g, uint32_t ng, cons
   if (n_ops) *n_ops = (uint32_t)plan.ops.size();
   if (n_passes) *n_passes = (uint32_t)plan.passes.size();
   if (tile_k) *tile_k = plan.k;
-  if (ops_out && max_ops >= plan.ops.size())
-    std::memcpy(ops_out, plan.ops.data(), sizeof(TileOp) * plan.ops.size());
-  if (spos_out && range_out && max_passes >= plan.passes.size()) {
+  if (ops_out && max_ops < plan.ops.size())
+    return set_error(QOA_ERR_ARG, "%zu ops do not fit in %u", plan.ops.size(), max_ops);
+  if ((spos_out || range_out) && max_passes < plan.passes.size())
+    return set_error(QOA_ERR_ARG, "%zu passes do not fit in %u", plan.passes.size(), max_passes);
+  if (ops_out) std::memcpy(ops_out, plan.ops.data(), sizeof(TileOp) * plan.ops.size());
+  if (spos_out && range_out) {
     for (size_t p = 0; p < plan.passes.size(); ++p) {
       std::memcpy(spos_out + 16 * p, plan.passes[p].spos, 16);
       range_out[3 * p] = plan.passes[p].op_begin;
