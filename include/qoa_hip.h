@@ -193,6 +193,15 @@ int qoa_probabilities(qoa_state* s, double* out, uint64_t n_amps);
    summed in a fixed order (chunks of 4096, runs of 16), so results are
    reproducible and independent of timing. The state is not collapsed. */
 int qoa_sample(qoa_state* s, uint64_t seed, uint32_t count, uint64_t* out);
+/* qoa_sample without any device allocation. No counterpart in the
+   reference: for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc). Its
+   scratch is a fixed per-device array of about 8.5 MiB, held until the
+   process ends, so it accepts states of at most 32 qubits and at most 65536
+   samples per call (QOA_ERR_ARG beyond). Same results as qoa_sample. Calls
+   that overlap on one GPU (from several threads, or several handles on one
+   device) share that scratch and race; calls in sequence do not. */
+int qoa_sample_noalloc(qoa_state* s, uint64_t seed, uint32_t count, uint64_t* out);
 /* One depolarizing event set: for every qubit q with code[q] in {1,2,3}
    apply X, Y or Z on q (0 = none), in qubit order, as one fused sweep
    (apply_depolarizing_noise, quantum_state.rs:872-885). */

@@ -217,6 +217,7 @@ _SIGS.update({
     "qoa_server_get_stats": ([P, C.POINTER(QoaServerStats)], I32),
     "qoa_server_destroy": ([P], I32),
     "qoa_sample": ([P, U64, U32, P], I32),
+    "qoa_sample_noalloc": ([P, U64, U32, P], I32),
 })
 
 

@@ -9,6 +9,8 @@
 // A sample with uniform u picks the first chunk, then run, then amplitude at
 // which the running sum exceeds u * total (if rounding leaves none, the last
 // one with non-zero weight). The CPU oracle mirrors this order exactly.
+#include <mutex>
+
 #include "qoa_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -236,5 +238,78 @@ extern "C" int qoa_sample(qoa_state* s, uint64_t seed, uint32_t count, uint64_t*
     rc = set_error(QOA_ERR_HIP, "sampling failed");
   hipFree(cs);
   hipFree(dout);
+  return rc;
+}
+
+// ---- allocation-free sampling ------------------------------------------------
+// qoa_sample_noalloc keeps its device scratch in one file-scope array per
+// device instead of allocating: the chunk running sums of a state of up to
+// kNoAllocMaxQubits qubits (2^(n-12) + 2 doubles) and up to kNoAllocMaxCount
+// samples. Every call on a device shares that device's copy; nothing in it
+// has to start at zero (each call writes every slot it reads).
+namespace qoa {
+namespace {
+constexpr uint32_t kNoAllocMaxQubits = 32;
+constexpr uint32_t kNoAllocMaxCount = 65536;
+constexpr uint64_t kNoAllocSumDoubles = (1ULL << (kNoAllocMaxQubits - 12)) + 2;
+constexpr uint64_t kNoAllocSampleOff = (kNoAllocSumDoubles + 31) / 32 * 32;  // 256-B aligned
+constexpr uint64_t kNoAllocDoubles = kNoAllocSampleOff + kNoAllocMaxCount;   // about 8.5 MiB
+}  // namespace
+}  // namespace qoa
+
+static __device__ __align__(256) double scratch[qoa::kNoAllocDoubles];
+
+namespace qoa {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (looked up once per device,
+// with that device current). In host code the array's name is only a
+// stand-in; kernels and copies take this address.
+int scratch_for(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return set_error(QOA_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    QOA_HIP_CHECK(hipGetDevice(&cur));
+    if (cur != device) QOA_HIP_CHECK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return set_error(QOA_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return QOA_OK;
+}
+}  // namespace
+}  // namespace qoa
+
+extern "C" int qoa_sample_noalloc(qoa_state* s, uint64_t seed, uint32_t count, uint64_t* out) {
+  using namespace qoa;
+  if (!valid(s)) return set_error(QOA_ERR_HANDLE, "invalid handle");
+  if (s->precision != QOA_C128) return set_error(QOA_ERR_UNSUPPORTED, "sampling needs QOA_C128");
+  if (!out && count) return set_error(QOA_ERR_ARG, "out is NULL");
+  if (s->n > kNoAllocMaxQubits)
+    return set_error(QOA_ERR_ARG, "qoa_sample_noalloc: %u qubits exceed its scratch (at most %u)",
+                     s->n, kNoAllocMaxQubits);
+  if (count > kNoAllocMaxCount)
+    return set_error(QOA_ERR_ARG, "qoa_sample_noalloc: %u samples exceed its scratch (at most %u)",
+                     count, kNoAllocMaxCount);
+  if (count == 0) return QOA_OK;
+  double* base = nullptr;
+  int rc = scratch_for(s->device, &base);
+  if (rc) return rc;
+  uint64_t* dout = reinterpret_cast<uint64_t*>(base + kNoAllocSampleOff);
+  rc = launch_sample(s->stream, (const double2*)s->amps, 0, s->n, 1, seed, count, nullptr, 0, base,
+                     dout);
+  if (rc == QOA_OK && hipMemcpyAsync(out, dout, sizeof(uint64_t) * count, hipMemcpyDeviceToHost,
+                                     s->stream) != hipSuccess)
+    rc = set_error(QOA_ERR_HIP, "sample copy failed");
+  if (hipStreamSynchronize(s->stream) != hipSuccess && rc == QOA_OK)
+    rc = set_error(QOA_ERR_HIP, "sampling failed");
   return rc;
 }
