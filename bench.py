@@ -45,7 +45,7 @@ FP32_MFMA_TFLOPS = 157.3
 FP64_TFLOPS = 78.6
 SIMDS = 1024          # 256 CUs x 4 SIMDs; a wave64 VALU or SALU instruction issues in 4 cycles
 CLOCK_MHZ = 2400.0    # peak engine clock
-QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA = 2, 4
+QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG = 2, 4, 16
 
 
 def log(msg: str) -> None:
@@ -218,6 +218,8 @@ def run_circuit(sv, gates, method: str, steps: int, warmup: int, tag: str):
             sv.run_fused(gates, flags=QOA_FUSE_LDS_ONLY)
         elif method == "fused-fma":
             sv.run_fused(gates, flags=QOA_FUSE_FMA)
+        elif method == "fused-merge":
+            sv.run_fused(gates, flags=QOA_FUSE_MERGE_DIAG)
         else:
             sv.run_fused(gates)
 
@@ -553,7 +555,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--method", default="fused",
-                    choices=["fused", "fused-fma", "fused-lds", "per-gate"])
+                    choices=["fused", "fused-fma", "fused-merge", "fused-lds", "per-gate"])
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=20241203)
@@ -618,7 +620,7 @@ def main() -> int:
         el, kk = run_circuit(ref, gates, "per-gate", 1, 0, "c2 per-gate")
         extras["c2_per-gate"] = {"ms_per_step": el * 1e3, "amp_updates_per_s": amp_updates / el,
                                  "kernels": kk}
-        for other in ("fused", "fused-fma", "fused-lds"):
+        for other in ("fused", "fused-fma", "fused-merge", "fused-lds"):
             el, kk = run_circuit(sv, gates, other, 1, 0, f"c2 {other}")
             d, f = sv.compare(ref)
             entry = {"ms_per_step": el * 1e3, "amp_updates_per_s": amp_updates / el,

@@ -93,6 +93,15 @@ typedef struct qoa_gate {
                                 FP64 instructions, results within a few ulp
                                 of the exact (num-complex) arithmetic
                                 instead of bit-identical to it.              */
+#define QOA_FUSE_MERGE_DIAG 16u /* register-resident passes merge each run of
+                                diagonal ops (Z/S/T/PHASE, CZ/CPHASE, RZ)
+                                into one op: per thread the factors that
+                                apply are multiplied together first, then
+                                into the amplitudes. Far fewer instructions
+                                for phase ladders; results within a few ulp
+                                (tested <= 1e-12) instead of bit-identical.
+                                Implies QOA_FUSE_FMA. Single-state fused
+                                runs only (batches and serving ignore it).  */
 #define QOA_CZ_REF_MASKS 1u  /* CZ/CPHASE use the integer c|t as the bit mask,
                                 as QuantumState::apply_cz does
                                 (src/runtime/quantum_state.rs:338, 356-361).
@@ -255,6 +264,13 @@ int qoa_plan_program(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
                      const double* matrices, const qoa_fuse_opts* opts, void* ops_out,
                      uint32_t max_ops, uint8_t* pass_spos_out, uint32_t* pass_range_out,
                      uint32_t max_passes, uint32_t* n_ops, uint32_t* n_passes, uint32_t* tile_k);
+
+/* The same plan's coefficient pool (host only): MAT4 matrices (32 doubles)
+   and merged diagonal terms (6 doubles each: rs, tl as two 32-bit words,
+   then gmask, gval, re, im, pad). NULL pool_out queries the size. */
+int qoa_plan_pool(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
+                  const double* matrices, const qoa_fuse_opts* opts, double* pool_out,
+                  uint32_t max_doubles, uint32_t* n_doubles);
 
 /* Dense 5-qubit blocks on complex64 states (the block-fusion primitive the
    boundary lists as apply_fused_block; no reference counterpart: the

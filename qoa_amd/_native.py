@@ -29,6 +29,7 @@ QOA_C64 = 1
 QOA_CZ_REF_MASKS = 1
 QOA_FUSE_LDS_ONLY = 2
 QOA_FUSE_FMA = 4
+QOA_FUSE_MERGE_DIAG = 16
 QOA_BLOCK_NO_RENORM = 8
 
 NOISE_NONE, NOISE_IDEAL, NOISE_RANDOM, NOISE_FIXED = 0, 1, 2, 3
@@ -123,6 +124,7 @@ _SIGS = {
     "qoa_plan_blocks": ([U32, P, U32, P, C.POINTER(QoaBlockOpts), P, P, U32, C.POINTER(U32)],
                         I32),
     "qoa_state_compare": ([P, P, DP, DP], I32),
+    "qoa_plan_pool": ([U32, P, U32, P, P, P, U32, C.POINTER(U32)], I32),
     "qoa_plan_program": ([U32, P, U32, P, P, P, U32, P, P, U32, C.POINTER(U32), C.POINTER(U32),
                           C.POINTER(U32)], I32),
     "qoa_timing_enable": ([P, I32], I32),

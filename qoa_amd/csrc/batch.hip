@@ -169,7 +169,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
     ops_base[j] = (uint32_t)all_ops.size();
     pool_base[j] = (uint32_t)all_pool.size();
     for (TileOp op : b->plans[j].ops) {
-      if (op.kind == TK_MAT4) op.aux += pool_base[j];
+      if (op.kind == TK_MAT4 || op.kind == TK_DIAGN) op.aux += pool_base[j];
       all_ops.push_back(op);
     }
     all_pool.insert(all_pool.end(), b->plans[j].pool.begin(), b->plans[j].pool.end());

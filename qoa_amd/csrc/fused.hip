@@ -260,7 +260,8 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
       // changes (half a 12-qubit tile)
       const unsigned threads = 1u << (k - kRegBits);
       const size_t lbytes = sizeof(double2) * reg_xchg_elems(k);
-      const bool fma = (opt.flags & QOA_FUSE_FMA) != 0;
+      // merged diagonal runs exist in the FMA instantiation only
+      const bool fma = (opt.flags & (QOA_FUSE_FMA | QOA_FUSE_MERGE_DIAG)) != 0;
       const dim3 grid((unsigned)tiles), block(threads);
       if (use32(s)) {
         if (fma)

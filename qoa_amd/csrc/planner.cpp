@@ -662,6 +662,101 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
   store_layout(&cur, k, swaps, out);
 }
 
+// ---- merged diagonal runs (QOA_FUSE_MERGE_DIAG) -------------------------------
+// VALU instructions of a register op on the 16 amplitudes of a thread (a
+// complex product is 6, a select 2) and of a TK_DIAGN op; the op dispatch
+// (about 40 scalar instructions, issued beside the vector work) is counted as
+// 20 vector instructions.
+constexpr int kDispatchCost = 20;
+inline int regs_of(uint32_t rs) { return kRegAmps >> __builtin_popcount(rs & 15u); }
+int diag_cost(const TileOp& op) {
+  switch (op.kind) {
+    case TK_DNEG: return op.tl ? 6 * regs_of(op.rsel) + 2 : 2 * regs_of(op.rsel);
+    case TK_DPHASE: return 6 * regs_of(op.rsel) + (op.tl ? 2 : 0);
+    default: return 6 * kRegAmps + (op.rt == 0xFF ? 2 : 0);  // TK_RZ
+  }
+}
+
+void diag_terms(const TileOp& op, std::vector<DiagTerm>* out) {
+  DiagTerm t{};
+  t.gmask = op.gmask;
+  t.gval = op.gval;
+  if (op.kind == TK_RZ) {  // m00 everywhere, then m11 / m00 where the bit is 1
+    t.rs = 0;
+    t.tl = 0;
+    t.c[0] = op.c[0];
+    t.c[1] = op.c[1];
+    out->push_back(t);
+    // m11 * conj(m00): |m00| = 1
+    t.c[0] = op.c[2] * op.c[0] + op.c[3] * op.c[1];
+    t.c[1] = op.c[3] * op.c[0] - op.c[2] * op.c[1];
+    if (op.rt != 0xFF) {
+      t.rs = 1u << op.rt;
+    } else {
+      t.tl = 1u << op.t;
+    }
+    out->push_back(t);
+    return;
+  }
+  t.rs = op.rsel;
+  t.tl = op.tl;
+  if (op.kind == TK_DNEG) {
+    t.c[0] = -1.0;
+    t.c[1] = 0.0;
+  } else {
+    t.c[0] = op.c[0];
+    t.c[1] = op.c[1];
+  }
+  out->push_back(t);
+}
+
+// Replaces runs of consecutive diagonal ops in ops[begin, end) by TK_DIAGN
+// ops where that costs fewer instructions. Every op of a run has the same
+// register layout (a layout op ends a run), so its rsel/tl fields hold.
+void merge_diag_runs(std::vector<TileOp>* ops, size_t begin, std::vector<double>* pool) {
+  auto diag = [](const TileOp& op) {
+    return op.kind == TK_DNEG || op.kind == TK_DPHASE || op.kind == TK_RZ;
+  };
+  std::vector<TileOp> out(ops->begin(), ops->begin() + (long)begin);
+  size_t i = begin;
+  while (i < ops->size()) {
+    if (!diag((*ops)[i])) {
+      out.push_back((*ops)[i++]);
+      continue;
+    }
+    size_t j = i;
+    int plain = 0;
+    std::vector<DiagTerm> terms;
+    while (j < ops->size() && diag((*ops)[j])) {
+      plain += diag_cost((*ops)[j]) + kDispatchCost;
+      diag_terms((*ops)[j], &terms);
+      ++j;
+    }
+    std::stable_sort(terms.begin(), terms.end(),
+                     [](const DiagTerm& a, const DiagTerm& b) { return a.rs < b.rs; });
+    int merged = kDispatchCost;
+    for (size_t t = 0; t < terms.size(); ++t) {
+      merged += terms[t].tl ? 8 : 6;
+      if (t == 0 || terms[t].rs != terms[t - 1].rs) merged += 6 * regs_of(terms[t].rs) + 4;
+    }
+    if (j - i < 2 || merged >= plain) {
+      for (size_t t = i; t < j; ++t) out.push_back((*ops)[t]);
+    } else {
+      TileOp m{};
+      m.kind = TK_DIAGN;
+      m.aux = (uint32_t)pool->size();
+      m.lmask = (uint32_t)terms.size();  // the term count
+      m.rt = 0xFF;
+      const size_t at = pool->size();
+      pool->resize(at + kDiagTermDoubles * terms.size());
+      std::memcpy(pool->data() + at, terms.data(), sizeof(DiagTerm) * terms.size());
+      out.push_back(m);
+    }
+    i = j;
+  }
+  ops->swap(out);
+}
+
 }  // namespace
 
 int canonical_gate(uint32_t n, const qoa_gate& g, const double* mats, uint32_t flags,
@@ -774,6 +869,8 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
     }
     if (pp.reg_mode) {
       segment_pass(pass_ops, pass_gi, k, (opt.flags & kPlanSwaps) != 0, &plan->ops, &plan->order);
+      if ((opt.flags & QOA_FUSE_MERGE_DIAG) && !(opt.flags & kPlanNoise))
+        merge_diag_runs(&plan->ops, pp.op_begin, &plan->pool);
     } else {
       plan->ops.insert(plan->ops.end(), pass_ops.begin(), pass_ops.end());
       for (size_t e = 0; e < pass_gi.size(); ++e)
@@ -826,6 +923,26 @@ extern "C" int qoa_plan_program(uint32_t n, const qoa_gate* g, uint32_t ng, cons
       range_out[3 * p + 2] = plan.passes[p].reg_mode;
     }
   }
+  return QOA_OK;
+}
+
+extern "C" int qoa_plan_pool(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
+                             const qoa_fuse_opts* opts, double* pool_out, uint32_t max_doubles,
+                             uint32_t* n_doubles) {
+  using namespace qoa;
+  if (!g && ng) return set_error(QOA_ERR_ARG, "gates is NULL");
+  if (!n_doubles) return set_error(QOA_ERR_ARG, "n_doubles is NULL");
+  PlanOptions po{opts ? opts->tile_qubits : 0u, opts ? opts->low_qubits : 0u,
+                 opts ? opts->flags : 0u};
+  FusedPlan plan;
+  int rc = build_plan(n, g, ng, mats, po, &plan);
+  if (rc) return rc;
+  *n_doubles = (uint32_t)plan.pool.size();
+  if (pool_out && max_doubles < plan.pool.size())
+    return set_error(QOA_ERR_ARG, "%zu pool doubles do not fit in %u", plan.pool.size(),
+                     max_doubles);
+  if (pool_out && !plan.pool.empty())
+    std::memcpy(pool_out, plan.pool.data(), sizeof(double) * plan.pool.size());
   return QOA_OK;
 }
 

@@ -18,9 +18,26 @@ enum TileKind : uint8_t {
   TK_RZ = 6,      // v *= bit t ? c[2..3] : c[0..1]
   TK_SWAP = 7,    // exchange (l|m2, l|m1) for l with both bits clear (and lmask set)
   TK_MAT4 = 8,    // 4x4 on local (t, t2) = (q1, q2); coefficients at pool[aux]
-  TK_LAYOUT = 9   // register passes: ins[0..3] = local bits of register slots 0..3;
+  TK_LAYOUT = 9,  // register passes: ins[0..3] = local bits of register slots 0..3;
                   // slots t < t2 keep their bit where the exchange needs it
+  TK_DIAGN = 10   // register passes (QOA_FUSE_MERGE_DIAG): a run of diagonal ops as
+                  // lmask terms at pool[aux] (DiagTerm), grouped by register slots
 };
+
+// One term of a TK_DIAGN op: the factor c applies where the tile condition
+// ((tile base ^ X frame) & gmask) == gval holds, the thread's local bits hold
+// tl, and the register index holds the slots rs. Terms come grouped by rs
+// (ascending); a group's factors are multiplied together (in term order)
+// per thread, then into the registers whose index holds rs. 48 bytes = 6
+// pool doubles.
+struct DiagTerm {
+  uint32_t rs, tl;
+  uint64_t gmask, gval;
+  double c[2];
+  uint32_t pad[2];
+};
+static_assert(sizeof(DiagTerm) == 48, "DiagTerm layout");
+constexpr uint32_t kDiagTermDoubles = sizeof(DiagTerm) / sizeof(double);
 
 // TK_M shapes (t2): general, real, or real diagonal with imaginary off-diagonal.
 constexpr uint8_t kM2General = 0, kM2Real = 1, kM2Rx = 2;

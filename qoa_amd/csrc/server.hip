@@ -276,7 +276,7 @@ bool admit(qoa_server* s, const std::shared_ptr<SrvJob>& j) {
   }
   std::vector<TileOp> ops = j->plan.ops;
   for (TileOp& op : ops)
-    if (op.kind == TK_MAT4) op.aux += (uint32_t)(pool_off / 8);
+    if (op.kind == TK_MAT4 || op.kind == TK_DIAGN) op.aux += (uint32_t)(pool_off / 8);
   if (!ops.empty())
     (void)hipMemcpyAsync(s->d_ops + oo, ops.data(), sizeof(TileOp) * ops.size(),
                          hipMemcpyHostToDevice, s->upload_stream);

@@ -1136,7 +1136,7 @@ __device__ __forceinline__ void load_coefs(OpRegs& r, const TileOp* __restrict__
 // the paths with thread controls.
 template <bool F>
 __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const RegLayout& L,
-                                       const double* __restrict__ pool) {
+                                       const double* __restrict__ pool, uint64_t tkey) {
   const uint32_t kind = op.kind();
   const uint32_t rsel = op.w[2] & 0xFFu;
   const uint32_t tl = op.w[3];
@@ -1202,6 +1202,34 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const
     case TK_MAT4:  // the planner puts the lower local bit in slot 2, the higher in slot 3
       reg_mat4<2, 3, F>(v, pool + op.aux());
       break;
+    case TK_DIAGN: {  // merged diagonal run: DiagTerm records, grouped by register slots
+      // (tolerance mode: only the FMA instantiation carries this code, so the
+      // exact kernels keep their register allocation)
+      if constexpr (!F) break;
+      const uint32_t* tw = reinterpret_cast<const uint32_t*>(pool + op.aux());
+      const uint32_t nt = op.w[9];
+      constexpr uint32_t W = kDiagTermDoubles * 2;  // words per term
+      uint32_t i = 0;
+      while (i < nt) {
+        const uint32_t rs = tw[W * i];
+        double2 acc = make_double2(1.0, 0.0);
+        for (; i < nt && tw[W * i] == rs; ++i) {
+          const uint32_t* w = tw + W * i;
+          const uint64_t gm = w[2] | (uint64_t)w[3] << 32, gv = w[4] | (uint64_t)w[5] << 32;
+          if ((tkey & gm) != gv) continue;  // uniform
+          const double cr = __builtin_bit_cast(double, w[6] | (uint64_t)w[7] << 32);
+          const double ci = __builtin_bit_cast(double, w[8] | (uint64_t)w[9] << 32);
+          const uint32_t ttl = w[1];
+          if (ttl == 0) {
+            ip_mulc<F, true>(acc, make_double2(cr, ci));
+          } else {
+            const bool on = (L.thr_l & ttl) == ttl;
+            ip_mulc<F>(acc, make_double2(on ? cr : 1.0, on ? ci : 0.0));
+          }
+        }
+        reg_phase_rs<F, false>(v, acc, rs);
+      }
+    } break;
     default:
       break;
   }
@@ -1318,7 +1346,7 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         }
       }
     } else if (wave_act && (((uint64_t)base ^ frame) & op.gmask()) == op.gval()) {  // uniform
-      reg_op<F>(v, op, L, pool);
+      reg_op<F>(v, op, L, pool, (uint64_t)base ^ frame);
     }
     const uint32_t nq = op.w[2] >> 24;
     if (NOISE && drawn) {  // trajectory errors after the gate, operand order

@@ -121,7 +121,11 @@ static void check_plan(const Circuit& c, const PlanOptions& po, const FusedPlan&
     }
     for (uint32_t o = pp.op_begin; o < pp.op_end && o < P.ops.size(); ++o) {
       const TileOp& op = P.ops[o];
-      EXPECT(op.kind <= TK_LAYOUT, "op kind %u", op.kind);
+      EXPECT(op.kind <= TK_DIAGN, "op kind %u", op.kind);
+      if (op.kind == TK_DIAGN) {
+        EXPECT(op.aux + kDiagTermDoubles * op.lmask <= P.pool.size(), "merged terms in the pool");
+        continue;
+      }
       if (op.kind == TK_LAYOUT) {
         for (int j = 0; j < kRegBits; ++j) EXPECT(op.ins[j] < pp.k, "layout bit %u", op.ins[j]);
         continue;
@@ -169,6 +173,7 @@ int main(int argc, char** argv) {
         PlanOptions po{tk, (uint32_t)(rng() % 4), 0u};
         if (variant & 1) po.flags |= QOA_CZ_REF_MASKS;
         if (variant == 2) po.flags |= QOA_FUSE_LDS_ONLY;
+        if (variant == 0 && (r % 3) == 0) po.flags |= QOA_FUSE_MERGE_DIAG;
         if (variant >= 3) po.flags |= kPlanForceReg;
         if (variant == 4) po.flags |= kPlanSwaps;
         if (variant == 5 && c.clean) po.flags |= kPlanSwaps | kPlanNoise;

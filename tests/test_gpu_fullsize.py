@@ -5,7 +5,8 @@ split over threads are the same arithmetic, tests/test_oracle_invariants.py).
 
   per-gate sweeps  bit-identical to the oracle (src/vectorization.rs:904-1261)
   fused passes     max |d amp| <= 1e-12 and 1 - F <= 1e-12 against the
-                   oracle in the circuit's own order (exact and FMA modes)
+                   oracle in the circuit's own order (exact, FMA and merged
+                   diagonal modes)
   batched jobs     bit-identical to the oracle in each job's planned order
 """
 from __future__ import annotations
@@ -20,7 +21,7 @@ import pytest
 import pyoracle as po
 from qoa_amd import StateVector
 from qoa_amd import workloads as W
-from qoa_amd._native import QOA_FUSE_FMA
+from qoa_amd._native import QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG
 from qoa_amd.batch import Batch
 
 pytestmark = pytest.mark.gpu
@@ -62,7 +63,8 @@ def test_c2_28_qubits_against_oracle():
         assert np.array_equal(out, ref), "per-gate sweeps differ from the oracle"
         del out
         say("per-gate: bit-identical")
-        for name, flags in (("exact", 0), ("fma", QOA_FUSE_FMA)):
+        for name, flags in (("exact", 0), ("fma", QOA_FUSE_FMA),
+                            ("merged diagonals", QOA_FUSE_MERGE_DIAG)):
             sv.apply_reset_all()
             sv.run_fused(g, flags=flags)
             d, inf = errors(sv.download(), ref)

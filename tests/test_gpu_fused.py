@@ -12,6 +12,7 @@ from qoa_amd._native import QOA_CZ_REF_MASKS
 
 QOA_FUSE_LDS_ONLY = 2
 QOA_FUSE_FMA = 4
+QOA_FUSE_MERGE_DIAG = 16
 MODES = [0, QOA_FUSE_LDS_ONLY]  # register-resident segments (default) / LDS-resident
 from test_planner import mixed_circuit
 
@@ -96,11 +97,13 @@ def test_index64_paths_match():
     assert outs[0][2] == outs[1][2] and outs[0][3] == outs[1][3]
 
 
+@pytest.mark.parametrize("flags", [QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG])
 @pytest.mark.parametrize("tile", [6, 8, 12])
-def test_fma_mode_within_tolerance(tile):
-    """FMA products: not bit-identical, but every amplitude within 1e-12 of
-    the oracle (original gate order) and fidelity >= 1 - 1e-12, on the
-    layered circuit and on the full gate surface (2x2, 4x4, controls)."""
+def test_fma_mode_within_tolerance(tile, flags):
+    """FMA products (and merged diagonal runs): not bit-identical, but every
+    amplitude within 1e-12 of the oracle (original gate order) and fidelity
+    >= 1 - 1e-12, on the layered circuit and on the full gate surface (2x2,
+    4x4, controls)."""
     n = 16
     g = W.layered_random(n, 20, seed=2, ladders=True)
     gm, mats = mixed_circuit(n, 400, seed=tile)
@@ -110,13 +113,13 @@ def test_fma_mode_within_tolerance(tile):
         po.run_gates(ref, gates, matrices=m)
         with StateVector(n) as sv:
             sv.upload(a0)
-            sv.run_fused(gates, tile_qubits=tile, flags=QOA_FUSE_FMA, matrices=m)
+            sv.run_fused(gates, tile_qubits=tile, flags=flags, matrices=m)
             out = sv.download()
         assert np.max(np.abs(out - ref)) <= 1e-12
         assert fidelity(out, ref) >= 1 - 1e-12
 
 
-@pytest.mark.parametrize("mode", MODES + [QOA_FUSE_FMA])
+@pytest.mark.parametrize("mode", MODES + [QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG])
 def test_c2_28q_fused_vs_per_gate_full_size(mode):
     """The full 28-qubit, 3164-gate circuit: fused passes against per-gate
     sweeps on the device (size-independent checks: unit norm, agreement
@@ -135,3 +138,23 @@ def test_c2_28q_fused_vs_per_gate_full_size(mode):
     del da, db
     assert d <= 1e-12, d
     assert f >= 1 - 1e-12, f
+
+
+def test_merged_diagonal_program_matches_emulator():
+    """The merged program on the device against the CPU emulation of the same
+    op program (the device multiplies with FMA, the emulator without: within
+    1e-13), at 13 qubits with 10-qubit tiles (ladders give long runs)."""
+    from tile_emulator import plan_pool, plan_program, run_program
+
+    n, tile = 13, 10
+    g = W.layered_random(n, 12, seed=9, ladders=True)
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=tile, flags=QOA_FUSE_MERGE_DIAG)
+    pool = plan_pool(n, g, tile_qubits=tile, flags=QOA_FUSE_MERGE_DIAG)
+    a0 = random_state(n, seed=3)
+    emu = a0.copy()
+    run_program(emu, n, ops, spos, rng, k, pool=pool)
+    with StateVector(n) as sv:
+        sv.upload(a0)
+        sv.run_fused(g, tile_qubits=tile, flags=QOA_FUSE_MERGE_DIAG)
+        out = sv.download()
+    assert np.max(np.abs(out - emu)) <= 1e-13

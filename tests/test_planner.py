@@ -7,7 +7,7 @@ import pyoracle as po
 from kat_util import random_state
 from qoa_amd import plan_fused
 from qoa_amd import workloads as W
-from qoa_amd._native import GATE_DTYPE, OP, QOA_CZ_REF_MASKS
+from qoa_amd._native import GATE_DTYPE, OP, QOA_CZ_REF_MASKS, QOA_FUSE_MERGE_DIAG
 
 
 def mixed_circuit(n: int, count: int, seed: int) -> tuple[np.ndarray, np.ndarray]:
@@ -128,6 +128,37 @@ def test_emulated_ansatz_and_layers(tile):
         a = po.state(n)
         run_program(a, n, ops, spos, rng, k)
         assert np.array_equal(a, ref)
+
+
+@pytest.mark.parametrize("n,tile,ladders", [(13, 10, True), (14, 12, True), (12, 8, False)])
+def test_merged_diagonal_runs_match_oracle(n, tile, ladders):
+    """QOA_FUSE_MERGE_DIAG: runs of diagonal ops become one op whose factors
+    are multiplied per register group first. The emulated program equals the
+    oracle in the planned order within 1e-12 (products regrouped, not
+    bit-identical), merged ops exist, and they carry every diagonal gate."""
+    from tile_emulator import TK_DIAGN, plan_pool, plan_program, run_program
+
+    if ladders:
+        g = W.layered_random(n, 10, seed=n, ladders=True)
+        mats = None
+    else:
+        g, mats = mixed_circuit(n, 300, seed=n)
+        g = g[g["op"] != OP["U4"]]
+    flags = QOA_FUSE_MERGE_DIAG
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=tile, flags=flags, matrices=mats)
+    pool = plan_pool(n, g, tile_qubits=tile, flags=flags, matrices=mats)
+    plain, _, _, _ = plan_program(n, g, tile_qubits=tile, matrices=mats)
+    merged = ops[ops["kind"] == TK_DIAGN]
+    assert len(merged) > 0 and len(ops) < len(plain)
+    order, _ = plan_fused(n, g, tile_qubits=tile, flags=flags)
+    ref = random_state(n, seed=n)
+    a = ref.copy()
+    po.run_gates(ref, g, matrices=mats, order=order)
+    run_program(a, n, ops, spos, rng, k, pool=pool)
+    assert np.max(np.abs(a - ref)) <= 1e-12
+    orig = random_state(n, seed=n)
+    po.run_gates(orig, g, matrices=mats)
+    assert np.max(np.abs(a - orig)) <= 1e-12
 
 
 def test_sincos_coefficients_match_oracle():

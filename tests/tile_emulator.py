@@ -21,7 +21,10 @@ TILE_OP = np.dtype([("kind", "u1"), ("t", "u1"), ("t2", "u1"), ("nins", "u1"), (
                     ("rsel", "u1"), ("rt", "u1"), ("outside", "u1"), ("nq", "u1"), ("tl", "<u4"),
                     ("gmask", "<u8"), ("gval", "<u8"), ("aux", "<u4"), ("lmask", "<u4"),
                     ("gi", "<u4"), ("psel", "u1"), ("qphys", "u1", 3), ("c", "<f8", 8)])
-TK_H, TK_M, TK_X, TK_Y, TK_DNEG, TK_DPHASE, TK_RZ, TK_SWAP, TK_MAT4, TK_LAYOUT = range(10)
+TK_H, TK_M, TK_X, TK_Y, TK_DNEG, TK_DPHASE, TK_RZ, TK_SWAP, TK_MAT4, TK_LAYOUT, TK_DIAGN = range(11)
+# a merged diagonal term: register slots, thread bits, tile condition, factor
+DIAG_TERM = np.dtype([("rs", "<u4"), ("tl", "<u4"), ("gmask", "<u8"), ("gval", "<u8"),
+                      ("c", "<f8", 2), ("pad", "<u4", 2)])
 
 
 def plan_program(n, gates, tile_qubits=0, low_qubits=0, flags=0, matrices=None):
@@ -44,6 +47,22 @@ def plan_program(n, gates, tile_qubits=0, low_qubits=0, flags=0, matrices=None):
               C.c_uint32(len(ops)), N.ptr(spos), N.ptr(rng), C.c_uint32(npass.value),
               C.byref(nops), C.byref(npass), C.byref(k)))
     return ops, spos, rng, k.value
+
+
+def plan_pool(n, gates, tile_qubits=0, low_qubits=0, flags=0, matrices=None) -> np.ndarray:
+    """The plan's coefficient pool (MAT4 matrices, merged diagonal terms)."""
+    lib = N.load()
+    f = lib.qoa_plan_pool
+    f.argtypes, f.restype = N._SIGS["qoa_plan_pool"]
+    g = gates_array(gates)
+    m = None if matrices is None else np.ascontiguousarray(matrices, dtype=np.complex128)
+    opts = N.QoaFuseOpts(tile_qubits, low_qubits, flags, 0)
+    cnt = C.c_uint32()
+    mp = None if m is None else N.ptr(m)
+    N.check(f(n, N.ptr(g), len(g), mp, C.byref(opts), None, 0, C.byref(cnt)))
+    pool = np.zeros(max(cnt.value, 1), dtype=np.float64)
+    N.check(f(n, N.ptr(g), len(g), mp, C.byref(opts), N.ptr(pool), pool.size, C.byref(cnt)))
+    return pool[: cnt.value]
 
 
 def _cmul(xr, xi, yr, yi):
@@ -70,8 +89,9 @@ def _check_reg_fields(op, kind: int, lm: int, layout) -> None:
         assert int(op["rt"]) == want, "RZ slot"
 
 
-def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int) -> None:
-    """Applies the program to the complex128 state `a` in place."""
+def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int, pool=None) -> None:
+    """Applies the program to the complex128 state `a` in place (`pool`: the
+    plan's coefficient pool, needed for merged diagonal ops)."""
     re, im = a.real.copy(), a.imag.copy()
     T = 1 << k
     loc = np.arange(T)
@@ -118,6 +138,33 @@ def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int) -> None:
                             assert new[j] == cur[j], "slot changed without a swap"
                     assert sorted(new) == sorted(set(new)), "slots hold distinct bits"
                     layout = new
+                    continue
+                if kind == TK_DIAGN:
+                    # merged diagonal run: per group of terms with the same
+                    # register slots rs, the product of the factors whose
+                    # tile and thread conditions hold, into the amplitudes
+                    # whose register bits hold rs
+                    assert reg_mode and pool is not None
+                    cnt, at = int(op["lmask"]), int(op["aux"])
+                    terms = pool[at:at + 6 * cnt].view(DIAG_TERM)
+                    assert len(terms) == cnt
+                    assert np.all(np.diff(terms["rs"].astype(np.int64)) >= 0), "terms grouped"
+                    R = sum(1 << b for b in layout)
+                    for rs in np.unique(terms["rs"]):
+                        ar, ai = np.ones(T), np.zeros(T)
+                        for t in terms[terms["rs"] == rs]:
+                            if (base & int(t["gmask"])) != int(t["gval"]):
+                                continue
+                            tl = int(t["tl"])
+                            assert tl & R == 0, "thread condition on a register bit"
+                            on = (loc & tl) == tl
+                            cr = np.where(on, t["c"][0], 1.0)
+                            ci = np.where(on, t["c"][1], 0.0)
+                            ar, ai = _cmul(ar, ai, cr, ci)
+                        need = sum(1 << layout[j] for j in range(4) if int(rs) >> j & 1)
+                        sel = (loc & need) == need
+                        r2, i2 = _cmul(vr, vi, ar, ai)
+                        vr, vi = np.where(sel, r2, vr), np.where(sel, i2, vi)
                     continue
                 lm = int(op["lmask"])
                 if reg_mode:
