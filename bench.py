@@ -338,25 +338,71 @@ def extra_c4(args) -> dict:
     t0 = time.perf_counter()
     jobs = [W.ansatz_job(args.seed + j) for j in range(1024)]
     gen = time.perf_counter() - t0
+    from qoa_amd._native import QOA_BATCH_AUTOTUNE, QOA_C64, load
+
+    lib = load()
+
+    def timed_runs(b, reps=3):
+        b.run()
+        b.synchronize()  # warm-up (and graph capture)
+        times = []
+        for _ in range(reps):
+            b.run()
+            times.append(b.synchronize())
+        return min(times)
+
+    pj = [(n, g) for n, _, g in jobs]
     t0 = time.perf_counter()
-    b = Batch([(n, g) for n, _, g in jobs], tile_qubits=12)
+    b = Batch(pj, tile_qubits=12, flags=QOA_BATCH_AUTOTUNE)
     plan_s = time.perf_counter() - t0
     info = b.info()
-    b.run()
-    b.synchronize()  # warm-up
-    times = []
-    for _ in range(3):
-        b.run()
-        times.append(b.synchronize())
-    ms = min(times)
+    ms = timed_runs(b)
+    tiles = {}
+    for j, (n, _, _) in enumerate(jobs):
+        tiles.setdefault(n, b.job_plan(j)[0])
+    # the same batch with every kernel launched from the host (no graph)
+    lib.qoa_set_option(b"batch_graphs", 0)
+    try:
+        with Batch(pj, tile_qubits=12, flags=QOA_BATCH_AUTOTUNE) as bd:
+            ms_direct = timed_runs(bd)
+    finally:
+        lib.qoa_set_option(b"batch_graphs", 1)
+    with Batch(pj, tile_qubits=12) as b12:
+        ms_12 = timed_runs(b12)
     amp_updates = float(sum(len(g) * (1 << n) for n, _, g in jobs))
     out = {"workload": "1024 hardware-efficient-ansatz circuits, n~U{8..24}, depth~U{10..60}, "
                        "FP64, all states resident", "jobs": 1024,
            "state_gib": info["total_amps"] * 16 / 2**30, "rounds": info["rounds"],
+           "launches": "one graph replay per run (the rounds' kernels captured once)",
            "ms_per_batch": ms, "jobs_per_s": 1024 / (ms * 1e-3),
            "amp_updates_per_s": amp_updates / (ms * 1e-3),
            "hbm_gbps": info["alg_bytes"] / (ms * 1e-3) / 1e9,
-           "gen_s": gen, "plan_s": plan_s}
+           "autotuned_tile_by_n": {str(k): v for k, v in sorted(tiles.items())},
+           "ms_per_batch_fixed_tile_12": ms_12,
+           "ms_per_batch_direct_launches": ms_direct,
+           "gen_s": gen, "plan_and_tune_s": plan_s}
+    # FP32-fused form: complex64 states, 5-qubit blocks on matrix cores
+    try:
+        t0 = time.perf_counter()
+        with Batch(pj, precision=QOA_C64) as bf:
+            fuse_s = time.perf_counter() - t0
+            fi = bf.info()
+            ms32 = timed_runs(bf, reps=2)
+            nblk = sum(bf.job_plan(j)[1] for j in range(1024))
+            sample = list(range(0, 1024, 64))
+            worst_inf, worst_d = 0.0, 0.0
+            for j in sample:
+                d, inf = compare_host(bf.download(j), b.download(j))
+                worst_d, worst_inf = max(worst_d, d), max(worst_inf, inf)
+        out["fp32_blocks"] = {
+            "method": "complex64 states, dense 5-qubit blocks (32x32, FP32 MFMA), round r = "
+                      "block r of every job", "ms_per_batch": ms32,
+            "jobs_per_s": 1024 / (ms32 * 1e-3), "blocks": nblk, "rounds": fi["rounds"],
+            "hbm_gbps": fi["alg_bytes"] / (ms32 * 1e-3) / 1e9, "fuse_s": fuse_s,
+            "vs_fp64_batch": {"jobs_compared": len(sample), "max_abs_diff": worst_d,
+                              "one_minus_fidelity_max": worst_inf}}
+    except Exception as e:  # noqa: BLE001
+        out["fp32_blocks"] = {"error": str(e)}
     if not args.no_plain:
         log("c4: plain per-gate per-job FP64 loop over the same 1024 jobs ...")
         pj = [(n, g) for n, _, g in jobs]
@@ -381,6 +427,9 @@ def extra_c4(args) -> dict:
             "method": "qoa_run_gates per job, one job after another (one sweep kernel per "
                       "gate, FP64), all 1024 jobs", "s": plain_s, "jobs_per_s": 1024 / plain_s}
         out["speedup_vs_plain_loop"] = (1024 / (ms * 1e-3)) / (1024 / plain_s)
+        if "jobs_per_s" in out.get("fp32_blocks", {}):
+            out["fp32_blocks"]["speedup_vs_plain_loop"] = \
+                out["fp32_blocks"]["jobs_per_s"] / (1024 / plain_s)
         out["accuracy_vs_plain_loop"] = {"jobs_compared": len(sample), "max_abs_diff": worst_d,
                                          "one_minus_fidelity_max": worst_inf}
     b.close()

@@ -113,7 +113,9 @@ const char* qoa_last_error(void);           /* thread-local, never NULL        *
 int qoa_device_info(char* name, size_t name_len, uint64_t* hbm_bytes, int* num_cus);
 /* Library options. "index64" (0/1): use 64-bit index arithmetic in every
    kernel even for states of at most 31 qubits (the path states of 32+ qubits
-   take), so that it can be tested on small states. */
+   take), so that it can be tested on small states. "batch_graphs" (0/1,
+   default 1): qoa_batch_run captures a batch's launch sequence once as a
+   HIP graph and replays it; 0 launches every kernel from the host. */
 int qoa_set_option(const char* key, int64_t value);
 
 /* ---- state lifecycle: QuantumState::new (quantum_state.rs:56-90) ------- */
@@ -184,6 +186,10 @@ int qoa_apply_toffoli(qoa_state* s, uint32_t c1, uint32_t c2, uint32_t t);
 int qoa_apply_controlled_y(qoa_state* s, uint32_t c, uint32_t t);
 
 /* ---- reductions and collapse (quantum_state.rs:128-133, 399-458, 888-907) */
+/* validate_state (quantum_state.rs:135-167) on the device: whether any
+   amplitude has a NaN / infinite component, and sum |a_i|^2; the caller
+   applies the reference's order of checks and its 1e-9 norm tolerance. */
+int qoa_validate(qoa_state* s, int* has_nan, int* has_inf, double* norm_sqr);
 /* sum_i |a_i|^2 (deterministic fixed-shape tree on the device). */
 int qoa_norm_sqr(qoa_state* s, double* out);
 /* normalize(): s = sum |a|^2; if s > 1e-12 divide every amplitude by sqrt(s),
@@ -340,10 +346,19 @@ typedef struct qoa_job_desc {
   uint32_t precision;    /* QOA_C128                                       */
   double p1, p2;         /* Pauli error prob after 1q / 2q gates (0 = none)*/
 } qoa_job_desc;
-/* Plans every job (fused register passes, tile = min(opts->tile_qubits or
-   12, n)) and allocates every instance's state in one device buffer. Fails
-   with QOA_ERR_OOM when the states do not fit. Gates use layer-1 semantics
-   (opts->flags may hold QOA_CZ_REF_MASKS). */
+/* Plans every job and allocates every instance's state in one device buffer
+   (QOA_ERR_OOM when they do not fit). Gates use layer-1 semantics
+   (opts->flags may hold QOA_CZ_REF_MASKS, QOA_BATCH_AUTOTUNE).
+   QOA_C128 jobs (the default): fused FP64 register passes, tile =
+   min(opts->tile_qubits or 12, n), trajectory noise allowed.
+   QOA_C64 jobs (all jobs of a batch share one precision): the FP32-fused
+   form, dense 5-qubit blocks (as qoa_run_blocks, no renormalisation) on FP32
+   matrix cores, round r applying block r of every job; noise-free jobs of
+   5..30 qubits. */
+#define QOA_BATCH_AUTOTUNE 32u /* qoa_fuse_opts.flags for qoa_batch_create:
+                                  pick each qubit count's tile (10, 11 or 12
+                                  qubits) by timing one of its jobs with each;
+                                  choices are cached for the process.        */
 int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* gates,
                      uint64_t n_gates_total, const double* matrices,
                      const qoa_fuse_opts* opts, qoa_batch** out);
@@ -355,8 +370,12 @@ int qoa_batch_synchronize(qoa_batch* b, double* run_ms);
 int qoa_batch_info(qoa_batch* b, uint32_t* n_rounds, uint64_t* total_amps, double* alg_bytes);
 int qoa_batch_download(qoa_batch* b, uint32_t job, uint32_t traj, double* amps,
                        uint64_t n_amps);
-/* Executed gate order of a job (for bit-exact checks against the oracle). */
+/* Executed gate order of a job (for bit-exact checks against the oracle;
+   QOA_C128 batches). */
 int qoa_batch_order(qoa_batch* b, uint32_t job, uint32_t* order, uint32_t n_order);
+/* A job's tile (QOA_C128: qubits per pass tile, after autotuning) or its
+   number of blocks (QOA_C64). */
+int qoa_batch_job_plan(qoa_batch* b, uint32_t job, uint32_t* tile_qubits, uint32_t* n_steps);
 int qoa_batch_destroy(qoa_batch* b);
 
 /* ---- serving: continuous batching of arriving jobs ------------------------

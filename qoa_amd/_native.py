@@ -30,6 +30,7 @@ QOA_CZ_REF_MASKS = 1
 QOA_FUSE_LDS_ONLY = 2
 QOA_FUSE_FMA = 4
 QOA_FUSE_MERGE_DIAG = 16
+QOA_BATCH_AUTOTUNE = 32
 QOA_BLOCK_NO_RENORM = 8
 
 NOISE_NONE, NOISE_IDEAL, NOISE_RANDOM, NOISE_FIXED = 0, 1, 2, 3
@@ -116,6 +117,7 @@ _SIGS = {
     "qoa_probabilities": ([P, P, U64], I32),
     "qoa_apply_pauli_codes": ([P, P, U32], I32),
     "qoa_run_gates": ([P, P, U32, P, U32], I32),
+    "qoa_validate": ([P, C.POINTER(I32), C.POINTER(I32), C.POINTER(D)], I32),
     "qoa_run_trajectory": ([P, P, U32, P, U32, U64, U32, D, D, P], I32),
     "qoa_run_fused": ([P, P, U32, P, P], I32),
     "qoa_plan_fused": ([U32, P, U32, P, P, C.POINTER(U32)], I32),
@@ -186,6 +188,7 @@ _SIGS.update({
     "qoa_batch_info": ([P, C.POINTER(U32), C.POINTER(U64), DP], I32),
     "qoa_batch_download": ([P, U32, U32, P, U64], I32),
     "qoa_batch_order": ([P, U32, P, U32], I32),
+    "qoa_batch_job_plan": ([P, U32, C.POINTER(U32), C.POINTER(U32)], I32),
     "qoa_batch_destroy": ([P], I32),
 })
 

@@ -12,12 +12,16 @@ from .state import gates_array
 
 
 class Batch:
-    """Jobs given as (n_qubits, gates) pairs; every job starts from |0...0>."""
+    """Jobs given as (n_qubits, gates) pairs; every job starts from |0...0>.
+
+    precision N.QOA_C128: fused FP64 register passes (noise allowed);
+    N.QOA_C64: the FP32-fused form, dense 5-qubit blocks on matrix cores.
+    flags may hold N.QOA_BATCH_AUTOTUNE (tiles chosen per qubit count)."""
 
     def __init__(self, jobs: Sequence[tuple[int, np.ndarray]], tile_qubits: int = 12,
                  low_qubits: int = 0, flags: int = 0, trajectories: int = 1,
                  matrices: np.ndarray | None = None, p1: float = 0.0, p2: float = 0.0,
-                 noise_seed: int | Sequence[int] = 0):
+                 noise_seed: int | Sequence[int] = 0, precision: int = N.QOA_C128):
         self._lib = load()
         arrays = [gates_array(g) for _, g in jobs]
         self.gates = np.ascontiguousarray(np.concatenate(arrays)) if arrays else \
@@ -27,7 +31,7 @@ class Batch:
         self.n = []
         for i, ((n, _), g) in enumerate(zip(jobs, arrays)):
             seed = noise_seed if isinstance(noise_seed, int) else int(noise_seed[i])
-            descs[i] = N.QoaJobDesc(int(n), len(g), off, seed, trajectories, N.QOA_C128, p1, p2)
+            descs[i] = N.QoaJobDesc(int(n), len(g), off, seed, trajectories, precision, p1, p2)
             off += len(g)
             self.n.append(int(n))
         self.n_gates = [len(g) for g in arrays]
@@ -74,6 +78,14 @@ class Batch:
         out = np.empty(1 << self.n[job], dtype=np.complex128)
         check(self._lib.qoa_batch_download(self._h, job, traj, ptr(out), out.size))
         return out
+
+    def job_plan(self, job: int) -> tuple[int, int]:
+        """(tile qubits, passes) of a C128 job; (5, blocks) of a C64 job."""
+        t, n = C.c_uint32(), C.c_uint32()
+        f = self._lib.qoa_batch_job_plan
+        f.argtypes, f.restype = N._SIGS["qoa_batch_job_plan"]
+        check(f(self._h, job, C.byref(t), C.byref(n)))
+        return t.value, n.value
 
     def order(self, job: int) -> np.ndarray:
         o = np.empty(self.n_gates[job], dtype=np.uint32)

@@ -30,6 +30,7 @@ int set_error(int code, const char* fmt, ...) {
 bool valid(const qoa_state* s) { return s != nullptr && s->magic == kMagic; }
 
 bool g_force_index64 = false;
+bool g_batch_graphs = true;
 
 const char* kind_name(int k) {
   static const char* names[KK_COUNT] = {"sweep_dense", "sweep_diag", "sweep_perm",
@@ -232,6 +233,10 @@ int qoa_set_option(const char* key, int64_t value) {
   if (!key) return set_error(QOA_ERR_ARG, "key is NULL");
   if (std::strcmp(key, "index64") == 0) {
     g_force_index64 = value != 0;
+    return QOA_OK;
+  }
+  if (std::strcmp(key, "batch_graphs") == 0) {
+    g_batch_graphs = value != 0;
     return QOA_OK;
   }
   return set_error(QOA_ERR_ARG, "unknown option '%s'", key);
@@ -569,6 +574,25 @@ int qoa_apply_controlled_y(qoa_state* s, uint32_t c, uint32_t t) {
 }
 
 // ---- reductions ---------------------------------------------------------------------
+int qoa_validate(qoa_state* s, int* has_nan, int* has_inf, double* norm_sqr) {
+  CHECK_STATE(s); CHECK_C128(s);
+  int rc = launch_nonfinite(s, 6);
+  if (rc == QOA_OK) rc = launch_norm_sum(s, 0, false, 0);
+  if (rc) return rc;
+  double slots[2];
+  QOA_HIP_CHECK(hipMemcpyAsync(&slots[0], s->red + kSlotBase, sizeof(double), hipMemcpyDeviceToHost,
+                               s->stream));
+  QOA_HIP_CHECK(hipMemcpyAsync(&slots[1], s->red + kSlotBase + 6, sizeof(double),
+                               hipMemcpyDeviceToHost, s->stream));
+  QOA_HIP_CHECK(hipStreamSynchronize(s->stream));
+  unsigned int flags;
+  std::memcpy(&flags, &slots[1], sizeof(flags));
+  if (has_nan) *has_nan = (flags & 1u) ? 1 : 0;
+  if (has_inf) *has_inf = (flags & 2u) ? 1 : 0;
+  if (norm_sqr) *norm_sqr = slots[0];
+  return QOA_OK;
+}
+
 int qoa_norm_sqr(qoa_state* s, double* out) {
   CHECK_STATE(s); CHECK_C128(s);
   if (!out) return set_error(QOA_ERR_ARG, "out is NULL");

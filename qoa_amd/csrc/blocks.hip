@@ -17,6 +17,7 @@
 #include <cstring>
 #include <vector>
 
+#include "block_device.hpp"
 #include "blocks_plan.hpp"
 #include "gates.hpp"
 #include "qoa_internal.hpp"
@@ -24,84 +25,13 @@
 namespace qoa {
 namespace {
 
-constexpr int kColGroups = 4;              // 32-column groups per wave
-constexpr int kBlockThreads = 256;         // 4 waves
-
 // ---- device -------------------------------------------------------------------
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-struct BlockArgs {
-  uint64_t ncols;  // 2^(n-5)
-  uint8_t q[kBlockQ];
-};
-
-template <typename I>
-__device__ __forceinline__ I deposit5(I x, const BlockArgs& ba) {
-#pragma unroll
-  for (int j = 0; j < kBlockQ; ++j) {
-    const int b = ba.q[j];
-    x = ((x >> b) << (b + 1)) | (x & ((I(1) << b) - I(1)));
-  }
-  return x;
-}
-
-template <typename I>
-__device__ __forceinline__ I rowoff(uint32_t r, const BlockArgs& ba) {
-  I o = 0;
-#pragma unroll
-  for (int j = 0; j < kBlockQ; ++j) o |= (I)((r >> j) & 1u) << ba.q[j];
-  return o;
-}
-
 template <typename I>
 __global__ __launch_bounds__(kBlockThreads) void k_block5(float2* __restrict__ a,
                                                          const float* __restrict__ w,
                                                          BlockArgs ba) {
   __shared__ float sw[kWFloats];
-  for (int i = threadIdx.x; i < kWFloats / 4; i += kBlockThreads)
-    reinterpret_cast<float4*>(sw)[i] = reinterpret_cast<const float4*>(w)[i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int h = lane >> 5, j = lane & 31;
-  const uint64_t wave_col0 =
-      ((uint64_t)blockIdx.x * (kBlockThreads / 64) + wave) * (uint64_t)(32 * kColGroups);
-  // the column groups of this wave, loads of group g+1 issued before the
-  // matrix products of group g
-  auto load_group = [&](int g, float2 (&x)[16], I& base, bool& ok) {
-    const uint64_t c = wave_col0 + (uint64_t)g * 32 + j;
-    ok = c < ba.ncols;
-    base = deposit5<I>((I)(ok ? c : 0), ba);
-#pragma unroll
-    for (int s = 0; s < 16; ++s)
-      x[s] = ok ? a[base | rowoff<I>((uint32_t)(2 * s + h), ba)] : make_float2(0.f, 0.f);
-  };
-  if (wave_col0 >= ba.ncols) return;  // uniform over the wave (no barrier follows)
-  float2 cur[16], nxt[16];
-  I base, nbase = 0;
-  bool ok, nok = false;
-  load_group(0, cur, base, ok);
-  for (int g = 0; g < kColGroups; ++g) {
-    const bool more = g + 1 < kColGroups && wave_col0 + (uint64_t)(g + 1) * 32 < ba.ncols;
-    if (more) load_group(g + 1, nxt, nbase, nok);
-    f32x16 acc0 = {}, acc1 = {};
-#pragma unroll
-    for (int s = 0; s < 32; ++s) {
-      const float b = s < 16 ? cur[s].x : cur[s - 16].y;
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(sw[s * 64 + lane], b, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(sw[(32 + s) * 64 + lane], b, acc1, 0, 0, 0);
-    }
-    if (ok) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r)
-        a[base | rowoff<I>((uint32_t)((r & 3) + 8 * (r >> 2) + 4 * h), ba)] =
-            make_float2(acc0[r], acc1[r]);
-    }
-    if (!more) break;
-#pragma unroll
-    for (int s = 0; s < 16; ++s) cur[s] = nxt[s];
-    base = nbase;
-    ok = nok;
-  }
+  block_apply<I>(a, w, sw, ba, blockIdx.x);
 }
 
 // Per-block partial sums of |a|^2 (fixed grid, fixed order).

@@ -33,6 +33,24 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return t;
 }
 
+// NaN / Inf anywhere in the state: bit 0 / bit 1 of *flags (cleared first by
+// k_clear_flags; one atomic OR per wave that saw any).
+__global__ void k_clear_flags(unsigned int* flags) { *flags = 0u; }
+
+template <typename I>
+__global__ __launch_bounds__(kBlock) void k_nonfinite(const double2* __restrict__ a, I len,
+                                                      unsigned int* __restrict__ flags) {
+  unsigned int f = 0u;
+  for (I i = (I)blockIdx.x * kBlock + threadIdx.x; i < len; i += (I)gridDim.x * kBlock) {
+    const double2 v = a[i];
+    if (__builtin_isnan(v.x) || __builtin_isnan(v.y)) f |= 1u;
+    if (__builtin_isinf(v.x) || __builtin_isinf(v.y)) f |= 2u;
+  }
+  const bool any_nan = __ballot((f & 1u) != 0u) != 0ULL, any_inf = __ballot((f & 2u) != 0u) != 0ULL;
+  if ((threadIdx.x & 63u) == 0u && (any_nan || any_inf))
+    atomicOr(flags, (any_nan ? 1u : 0u) | (any_inf ? 2u : 0u));
+}
+
 // sum |a_i|^2 split by bit `selbit` (selbit == 0: everything goes to acc0).
 template <typename I>
 __global__ __launch_bounds__(kBlock) void k_norm_partials(const double2* __restrict__ a, I len,
@@ -209,6 +227,24 @@ inline unsigned grid1(uint64_t count) {
 }
 
 }  // namespace
+
+int launch_nonfinite(qoa_state* s, int slot) {
+  unsigned int* flags = reinterpret_cast<unsigned int*>(s->red + kSlotBase + slot);
+  const double2* a = (const double2*)s->amps;
+  const uint64_t nb64 = (s->len + kBlock - 1) / kBlock;
+  const unsigned nb = (unsigned)(nb64 < (uint64_t)kRedBlocks ? nb64 : kRedBlocks);
+  hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(1), 0, s->stream, flags);
+  timing_begin(s, KK_REDUCE, 16.0 * (double)s->len);
+  if (use32(s))
+    hipLaunchKernelGGL((k_nonfinite<uint32_t>), dim3(nb), dim3(kBlock), 0, s->stream, a,
+                       (uint32_t)s->len, flags);
+  else
+    hipLaunchKernelGGL((k_nonfinite<uint64_t>), dim3(nb), dim3(kBlock), 0, s->stream, a,
+                       (uint64_t)s->len, flags);
+  timing_end(s);
+  QOA_HIP_CHECK(hipGetLastError());
+  return QOA_OK;
+}
 
 int launch_norm_sum(qoa_state* s, uint64_t selbit, bool two_way, int slot) {
   const double2* a = (const double2*)s->amps;

@@ -92,6 +92,9 @@ bool valid(const qoa_state* s);
 
 // 32-bit index arithmetic unless the state needs more (or it is forced off).
 extern bool g_force_index64;
+// Batches replay their launch sequence as a captured HIP graph (option
+// "batch_graphs", default on).
+extern bool g_batch_graphs;
 inline bool use32(const qoa_state* s) { return s->n <= 31 && !g_force_index64; }
 
 // launch bracketing for timing
@@ -121,6 +124,9 @@ int launch_set_basis0(qoa_state* s);
 // slot `slot`; with two_way, the complementary sum goes into slot+1.
 int launch_norm_sum(qoa_state* s, uint64_t selbit, bool two_way, int slot);
 int launch_normalize_apply(qoa_state* s, int slot);
+// NaN (bit 0) / Inf (bit 1) flags of the state into result slot `slot`
+// (read as an unsigned int).
+int launch_nonfinite(qoa_state* s, int slot);
 int launch_collapse(qoa_state* s, uint64_t bit, int outcome, double factor);
 int launch_reset_apply(qoa_state* s, uint64_t bit, int slot);
 int launch_probabilities(qoa_state* s, double* dev_out);

@@ -375,15 +375,16 @@ class QuantumState(StateVector):
         return self.probabilities()
 
     def validate_state(self) -> None:
-        """quantum_state.rs:135-167 (raises ValueError)."""
-        a = self.download()
-        if np.isnan(a.real).any() or np.isnan(a.imag).any():
+        """quantum_state.rs:135-167 (raises ValueError); one device pass, no
+        copy of the state to the host."""
+        nan, inf, s = C.c_int(), C.c_int(), C.c_double()
+        check(self._lib.qoa_validate(self._h, C.byref(nan), C.byref(inf), C.byref(s)))
+        if nan.value:
             raise ValueError("quantum state contains nan values.")
-        if np.isinf(a.real).any() or np.isinf(a.imag).any():
+        if inf.value:
             raise ValueError("quantum state contains infinite values.")
-        s = self.norm_sqr()
-        if abs(s - 1.0) > 1e-9:
-            raise ValueError(f"quantum state is not normalized. norm squared: {s}")
+        if abs(s.value - 1.0) > 1e-9:
+            raise ValueError(f"quantum state is not normalized. norm squared: {s.value}")
 
     def get_state_vector(self) -> np.ndarray:
         return self.download()

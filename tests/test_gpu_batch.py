@@ -120,3 +120,79 @@ def test_noisy_mixed_surface_matches_oracle(tile):
                 ref = po.state(n)
                 po.run_trajectory(ref, g, seeds[j], t, p1, p2, matrices=table, order=order)
                 assert np.array_equal(b.download(j, t), ref), (tile, j, n, t)
+
+
+def test_graph_replays_equal_direct_launches():
+    """The batch's launch sequence captured as a graph: several replays, with
+    downloads and synchronisations between them, equal the run launched
+    kernel by kernel (and the oracle in planned order)."""
+    from qoa_amd._native import load
+
+    lib = load()
+    jobs = []
+    for j in range(12):
+        n, depth, g = W.ansatz_job(1300 + j, n_min=6, n_max=15, d_min=3, d_max=10)
+        jobs.append((n, g))
+    outs = {}
+    for graphs in (0, 1):
+        assert lib.qoa_set_option(b"batch_graphs", graphs) == 0
+        try:
+            with Batch(jobs, tile_qubits=10) as b:
+                runs = []
+                for _ in range(3):
+                    b.run()
+                    b.synchronize()
+                    runs.append([b.download(j) for j in range(len(jobs))])
+                for r in runs[1:]:
+                    for j in range(len(jobs)):
+                        assert np.array_equal(r[j], runs[0][j]), (graphs, j)
+                outs[graphs] = runs[0]
+                for j, (n, g) in enumerate(jobs):
+                    ref = po.state(n)
+                    po.run_gates(ref, g, order=b.order(j))
+                    assert np.array_equal(runs[0][j], ref), (graphs, j)
+        finally:
+            lib.qoa_set_option(b"batch_graphs", 1)
+    for j in range(len(jobs)):
+        assert np.array_equal(outs[0][j], outs[1][j])
+
+
+def test_autotuned_tiles_are_valid_and_exact():
+    from qoa_amd._native import QOA_BATCH_AUTOTUNE
+
+    jobs = []
+    for j in range(10):
+        n, depth, g = W.ansatz_job(1400 + j, n_min=9, n_max=16, d_min=4, d_max=12)
+        jobs.append((n, g))
+    with Batch(jobs, tile_qubits=12, flags=QOA_BATCH_AUTOTUNE) as b:
+        b.run()
+        b.synchronize()
+        for j, (n, g) in enumerate(jobs):
+            tile, passes = b.job_plan(j)
+            assert (tile == n if n <= 10 else 10 <= tile <= min(12, n)) and passes >= 1
+            ref = po.state(n)
+            po.run_gates(ref, g, order=b.order(j))
+            assert np.array_equal(b.download(j), ref), (j, n, tile)
+
+
+def test_fp32_block_batch_within_tolerance():
+    """The FP32-fused batch (5-qubit blocks on matrix cores, complex64 states):
+    every job within 1 - F <= 1e-5 and max |d amp| <= 1e-5 of the oracle's
+    FP64 run; a second run starts again from |0...0>."""
+    from qoa_amd._native import QOA_C64
+
+    jobs = []
+    for j in range(20):
+        n, depth, g = W.ansatz_job(1500 + j, n_min=5, n_max=16, d_min=3, d_max=20)
+        jobs.append((n, g))
+    with Batch(jobs, precision=QOA_C64) as b:
+        for rep in range(2):
+            b.run()
+            b.synchronize()
+            for j, (n, g) in enumerate(jobs):
+                out = b.download(j)
+                ref = po.state(n)
+                po.run_gates(ref, g)
+                f = abs(np.vdot(ref, out)) ** 2 / (np.vdot(ref, ref).real * np.vdot(out, out).real)
+                assert 1 - f <= 1e-5 and np.max(np.abs(out - ref)) <= 1e-5, (rep, j, n, 1 - f)
+                assert b.job_plan(j)[0] == 5 and b.job_plan(j)[1] >= 1

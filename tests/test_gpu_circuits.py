@@ -164,3 +164,26 @@ def test_measure_all_leaves_state():
     assert len(bits) == 5 and set(bits) <= {0, 1}
     assert np.array_equal(qs.download(), before)
     qs.close()
+
+
+def test_validate_state_on_device():
+    """validate_state (quantum_state.rs:135-167) with the checks on the device:
+    NaN before Inf before the 1e-9 norm tolerance, messages as the reference."""
+    from qoa_amd import QuantumState
+
+    n = 12
+    with QuantumState(n) as qs:
+        qs.validate_state()
+        a = random_state(n, seed=1)
+        for bad, msg in ((np.nan, "nan"), (np.inf, "infinite")):
+            b = a.copy()
+            b[1234] = complex(bad, 0.0) if msg == "nan" else complex(0.0, bad)
+            b[99] = complex(np.inf, 0.0) if msg == "nan" else b[99]  # NaN is reported first
+            qs.upload(b)
+            with pytest.raises(ValueError, match=msg):
+                qs.validate_state()
+        qs.upload(a * 1.001)
+        with pytest.raises(ValueError, match="not normalized"):
+            qs.validate_state()
+        qs.upload(a)
+        qs.validate_state()
