@@ -48,6 +48,18 @@ CLOCK_MHZ = 2400.0    # peak engine clock
 QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG = 2, 4, 16
 
 
+METHOD_NOTES = {
+    "fused-merge": "fused register passes (12-qubit tiles), FP64 with fused multiply-adds, "
+                   "runs of diagonal gates merged per register group; within 1e-12 of the "
+                   "per-gate FP64 sweeps (accuracy field)",
+    "fused": "fused register passes, FP64 without FMA: bit-identical to the oracle in the "
+             "planned order",
+    "fused-fma": "fused register passes, FP64 with fused multiply-adds",
+    "fused-lds": "fused passes with the tile in LDS",
+    "per-gate": "one sweep kernel per gate (bit-identical to the oracle)",
+}
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -603,7 +615,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--method", default="fused",
+    ap.add_argument("--method", default="fused-merge",
                     choices=["fused", "fused-fma", "fused-merge", "fused-lds", "per-gate"])
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--depth", type=int, default=40)
@@ -750,7 +762,8 @@ def main() -> int:
         "config": {"workload": f"{n}-qubit layered random circuit, depth {args.depth}, CPHASE "
                                f"ladders k=1..8 every 5th layer ({len(gates)} gates), complex128 "
                                "state resident in HBM", "n_qubits": n, "depth": args.depth,
-                   "gates": len(gates), "seed": args.seed, "method": args.method},
+                   "gates": len(gates), "seed": args.seed, "method": args.method,
+                   "method_note": METHOD_NOTES.get(args.method, "")},
         "accuracy": accuracy,
         "roofline": roofline(dom, args, copy),
         "hbm_gbps_all_kernels": tot_b / (tot_ms * 1e-3) / 1e9 if tot_ms else None,
@@ -782,15 +795,17 @@ def roofline(dom, args, copy=None) -> dict:
         out["copy_ceiling"] = copy
         out["frac_of_copy"] = k["gbps"] / copy["gbps"]
     default = (args.qubits, args.depth, args.seed) == (28, 40, 20241203)
-    prof = measured_profile("fused_pass/28q-d40-ladders") if (
-        default and kind == "fused_pass" and args.method == "fused") else None
+    key = {"fused": "fused_pass/28q-d40-ladders",
+           "fused-merge": "fused_pass_merge/28q-d40-ladders"}.get(args.method)
+    prof = measured_profile(key) if (default and kind == "fused_pass" and key) else None
     if prof:
         out["traffic"] = prof["bytes_per_launch"]
         out["traffic_source"] = "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, " \
                                 "copy-calibrated)"
         # the pass is bound by instruction issue, not HBM: its FP64 rate and the
         # share of the pass's cycles each SIMD spends issuing VALU / SALU work
-        fl = prof["fp64_valu_instr"] * 64 / (k["avg_us"] * 1e-6) / 1e12
+        flop = prof.get("fp64_flop", prof["fp64_valu_instr"] * 64)  # an FMA is two flops
+        fl = flop / (k["avg_us"] * 1e-6) / 1e12
         out["fp64_valu"] = {"achieved_tflops": fl, "peak_tflops": FP64_TFLOPS,
                             "frac": fl / FP64_TFLOPS,
                             "fp64_share_of_valu_instr": prof["fp64_valu_instr"] / prof["valu_instr"]}

@@ -187,3 +187,22 @@ def test_validate_state_on_device():
             qs.validate_state()
         qs.upload(a)
         qs.validate_state()
+
+
+def test_state_json_device_round_trip(tmp_path):
+    """F2 (src/main.rs:3800-3827, 4290-4296): a device state saved as the serde
+    document and loaded back into a new device state is bit-identical, with
+    its noise configuration and registers."""
+    from qoa_amd import NoiseConfig, stateio
+
+    n = 10
+    with QuantumState(n, NoiseConfig.Fixed(0.02)) as qs:
+        qs.run_gates(W.layered_random(n, 6, seed=11, ladders=True))
+        before = qs.download()
+        path = tmp_path / "state.json"
+        stateio.save_state(str(path), qs, registers=[1.5, -2.0])
+    qs2, regs, meta = stateio.load_state(str(path))
+    with qs2:
+        assert qs2.n == n and regs == [1.5, -2.0]
+        assert qs2.get_noise_config().kind == "Fixed" and qs2.get_noise_config().p == 0.02
+        assert np.array_equal(qs2.download(), before)
