@@ -29,6 +29,15 @@ constexpr int kBlock = 256;  // 4 wavefronts of 64
 #endif
 constexpr int kItems = QOA_SWEEP_ITEMS;
 
+// Workgroups walk the blocks of kBlock * kItems elements in strides of the
+// grid (uniform per workgroup, so cross-lane shuffles stay in step): the
+// grid is capped below so that the dispatch's work-item count fits 32 bits
+// (a 32-qubit whole-state sweep has 2^24 blocks of 256).
+#define FOR_EACH_BLOCK(count)                                                       \
+  for (I blk_ = (I)blockIdx.x, nblk_ = ((count) + (I)(kBlock * kItems) - 1) /      \
+                                          (I)(kBlock * kItems);                   \
+       blk_ < nblk_; blk_ += (I)gridDim.x)
+
 struct Ins {                 // fixed bit positions, ascending
   int n;
   uint8_t pos[44];
@@ -77,26 +86,28 @@ struct OpY {  // apply_y_simd (vectorization.rs:980-981)
 template <typename I, class Op>
 __global__ __launch_bounds__(kBlock) void k_pair(double2* __restrict__ a, I npairs, Ins ins,
                                                  I tbit, I cmask, Op op) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
-  I i0[kItems];
-  double2 x[kItems], y[kItems];
+  FOR_EACH_BLOCK(npairs) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
+    I i0[kItems];
+    double2 x[kItems], y[kItems];
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I p = base + (I)(j * kBlock);
-    i0[j] = deposit(p, ins) | cmask;
-    if (p < npairs) {
-      x[j] = a[i0[j]];
-      y[j] = a[i0[j] | tbit];
+    for (int j = 0; j < kItems; ++j) {
+      const I p = base + (I)(j * kBlock);
+      i0[j] = deposit(p, ins) | cmask;
+      if (p < npairs) {
+        x[j] = a[i0[j]];
+        y[j] = a[i0[j] | tbit];
+      }
     }
-  }
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I p = base + (I)(j * kBlock);
-    if (p < npairs) {
-      const double2 o0 = op.out0(x[j], y[j]);
-      const double2 o1 = op.out1(x[j], y[j]);
-      a[i0[j]] = o0;
-      a[i0[j] | tbit] = o1;
+    for (int j = 0; j < kItems; ++j) {
+      const I p = base + (I)(j * kBlock);
+      if (p < npairs) {
+        const double2 o0 = op.out0(x[j], y[j]);
+        const double2 o1 = op.out1(x[j], y[j]);
+        a[i0[j]] = o0;
+        a[i0[j] | tbit] = o1;
+      }
     }
   }
 }
@@ -105,49 +116,53 @@ __global__ __launch_bounds__(kBlock) void k_pair(double2* __restrict__ a, I npai
 template <typename I, class Op>
 __global__ __launch_bounds__(kBlock) void k_shfl(double2* __restrict__ a, I nelem, Ins ins,
                                                  I cmask, int tv, Op op) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
-  const int hi = (threadIdx.x >> tv) & 1;
-  const int lm = 1 << tv;
-  I e[kItems];
-  double2 v[kItems];
+  FOR_EACH_BLOCK(nelem) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
+    const int hi = (threadIdx.x >> tv) & 1;
+    const int lm = 1 << tv;
+    I e[kItems];
+    double2 v[kItems];
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I p = base + (I)(j * kBlock);
-    e[j] = deposit(p, ins) | cmask;
-    v[j] = make_double2(0.0, 0.0);
-    if (p < nelem) v[j] = a[e[j]];
-  }
+    for (int j = 0; j < kItems; ++j) {
+      const I p = base + (I)(j * kBlock);
+      e[j] = deposit(p, ins) | cmask;
+      v[j] = make_double2(0.0, 0.0);
+      if (p < nelem) v[j] = a[e[j]];
+    }
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    double2 o;
-    o.x = __shfl_xor(v[j].x, lm);
-    o.y = __shfl_xor(v[j].y, lm);
-    const double2 r = hi ? op.out1(o, v[j]) : op.out0(v[j], o);
-    const I p = base + (I)(j * kBlock);
-    if (p < nelem) a[e[j]] = r;
+    for (int j = 0; j < kItems; ++j) {
+      double2 o;
+      o.x = __shfl_xor(v[j].x, lm);
+      o.y = __shfl_xor(v[j].y, lm);
+      const double2 r = hi ? op.out1(o, v[j]) : op.out0(v[j], o);
+      const I p = base + (I)(j * kBlock);
+      if (p < nelem) a[e[j]] = r;
+    }
   }
 }
 
 template <typename I, bool NEG>
 __global__ __launch_bounds__(kBlock) void k_diag(double2* __restrict__ a, I count, Ins ins,
                                                  I mask, double2 f) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
-  I e[kItems];
-  double2 v[kItems];
+  FOR_EACH_BLOCK(count) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
+    I e[kItems];
+    double2 v[kItems];
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I p = base + (I)(j * kBlock);
-    e[j] = deposit(p, ins) | mask;
-    if (p < count) v[j] = a[e[j]];
-  }
+    for (int j = 0; j < kItems; ++j) {
+      const I p = base + (I)(j * kBlock);
+      e[j] = deposit(p, ins) | mask;
+      if (p < count) v[j] = a[e[j]];
+    }
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I p = base + (I)(j * kBlock);
-    if (p < count) {
-      double2 r;
-      if (NEG) r = make_double2(v[j].x * -1.0, v[j].y * -1.0);  // Complex::scale(-1.0)
-      else r = cmul(v[j], f);
-      a[e[j]] = r;
+    for (int j = 0; j < kItems; ++j) {
+      const I p = base + (I)(j * kBlock);
+      if (p < count) {
+        double2 r;
+        if (NEG) r = make_double2(v[j].x * -1.0, v[j].y * -1.0);  // Complex::scale(-1.0)
+        else r = cmul(v[j], f);
+        a[e[j]] = r;
+      }
     }
   }
 }
@@ -155,17 +170,19 @@ __global__ __launch_bounds__(kBlock) void k_diag(double2* __restrict__ a, I coun
 template <typename I>
 __global__ __launch_bounds__(kBlock) void k_rz(double2* __restrict__ a, I len, I bit, double2 m00,
                                                double2 m11) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
-  double2 v[kItems];
+  FOR_EACH_BLOCK(len) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
+    double2 v[kItems];
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I i = base + (I)(j * kBlock);
-    if (i < len) v[j] = a[i];
-  }
+    for (int j = 0; j < kItems; ++j) {
+      const I i = base + (I)(j * kBlock);
+      if (i < len) v[j] = a[i];
+    }
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I i = base + (I)(j * kBlock);
-    if (i < len) a[i] = (i & bit) ? cmul(v[j], m11) : cmul(v[j], m00);
+    for (int j = 0; j < kItems; ++j) {
+      const I i = base + (I)(j * kBlock);
+      if (i < len) a[i] = (i & bit) ? cmul(v[j], m11) : cmul(v[j], m00);
+    }
   }
 }
 
@@ -173,15 +190,17 @@ __global__ __launch_bounds__(kBlock) void k_rz(double2* __restrict__ a, I len, I
 template <typename I>
 __global__ __launch_bounds__(kBlock) void k_swap2(double2* __restrict__ a, I count, Ins ins,
                                                   I cmask, I m1, I m2) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
+  FOR_EACH_BLOCK(count) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I p = base + (I)(j * kBlock);
-    if (p < count) {
-      const I b = deposit(p, ins) | cmask;
-      const double2 u = a[b | m2], w = a[b | m1];
-      a[b | m2] = w;
-      a[b | m1] = u;
+    for (int j = 0; j < kItems; ++j) {
+      const I p = base + (I)(j * kBlock);
+      if (p < count) {
+        const I b = deposit(p, ins) | cmask;
+        const double2 u = a[b | m2], w = a[b | m1];
+        a[b | m2] = w;
+        a[b | m1] = u;
+      }
     }
   }
 }
@@ -195,23 +214,25 @@ struct Mat4 {
 template <typename I>
 __global__ __launch_bounds__(kBlock) void k_mat4(double2* __restrict__ a, I count, Ins ins, I ml,
                                                  I mh, Mat4 u) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
+  FOR_EACH_BLOCK(count) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
 #pragma unroll 1
-  for (int j = 0; j < kItems; ++j) {
-    const I p = base + (I)(j * kBlock);
-    if (p < count) {
-      const I b = deposit(p, ins);
-      const I idx[4] = {b, b | ml, b | mh, b | ml | mh};
-      double2 v[4];
+    for (int j = 0; j < kItems; ++j) {
+      const I p = base + (I)(j * kBlock);
+      if (p < count) {
+        const I b = deposit(p, ins);
+        const I idx[4] = {b, b | ml, b | mh, b | ml | mh};
+        double2 v[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) v[k] = a[idx[k]];
+        for (int k = 0; k < 4; ++k) v[k] = a[idx[k]];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        double2 acc = cmul(make_double2(u.m[8 * s], u.m[8 * s + 1]), v[0]);
+        for (int s = 0; s < 4; ++s) {
+          double2 acc = cmul(make_double2(u.m[8 * s], u.m[8 * s + 1]), v[0]);
 #pragma unroll
-        for (int k = 1; k < 4; ++k)
-          acc = cadd(acc, cmul(make_double2(u.m[8 * s + 2 * k], u.m[8 * s + 2 * k + 1]), v[k]));
-        a[idx[s]] = acc;
+          for (int k = 1; k < 4; ++k)
+            acc = cadd(acc, cmul(make_double2(u.m[8 * s + 2 * k], u.m[8 * s + 2 * k + 1]), v[k]));
+          a[idx[s]] = acc;
+        }
       }
     }
   }
@@ -248,19 +269,21 @@ __device__ __forceinline__ I pauli_chain(I idx, double2& v, const PauliList& pl)
 template <typename I>
 __global__ __launch_bounds__(kBlock) void k_pauli_pairs(double2* __restrict__ a, I npairs, int low,
                                                         I xmask, PauliList pl) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
+  FOR_EACH_BLOCK(npairs) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
 #pragma unroll 1
-  for (int j = 0; j < kItems; ++j) {
-    const I p = base + (I)(j * kBlock);
-    if (p < npairs) {
-      const I lo = p & ((I(1) << low) - I(1));
-      const I i = ((p >> low) << (low + 1)) | lo;
-      const I k = i ^ xmask;
-      double2 vi = a[i], vk = a[k];
-      const I di = pauli_chain(i, vi, pl);
-      const I dk = pauli_chain(k, vk, pl);
-      a[di] = vi;
-      a[dk] = vk;
+    for (int j = 0; j < kItems; ++j) {
+      const I p = base + (I)(j * kBlock);
+      if (p < npairs) {
+        const I lo = p & ((I(1) << low) - I(1));
+        const I i = ((p >> low) << (low + 1)) | lo;
+        const I k = i ^ xmask;
+        double2 vi = a[i], vk = a[k];
+        const I di = pauli_chain(i, vi, pl);
+        const I dk = pauli_chain(k, vk, pl);
+        a[di] = vi;
+        a[dk] = vk;
+      }
     }
   }
 }
@@ -268,31 +291,37 @@ __global__ __launch_bounds__(kBlock) void k_pauli_pairs(double2* __restrict__ a,
 template <typename I>
 __global__ __launch_bounds__(kBlock) void k_pauli_diag(double2* __restrict__ a, I len,
                                                        PauliList pl) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
+  FOR_EACH_BLOCK(len) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I i = base + (I)(j * kBlock);
-    if (i < len) {
-      double2 v = a[i];
-      pauli_chain(i, v, pl);
-      a[i] = v;
+    for (int j = 0; j < kItems; ++j) {
+      const I i = base + (I)(j * kBlock);
+      if (i < len) {
+        double2 v = a[i];
+        pauli_chain(i, v, pl);
+        a[i] = v;
+      }
     }
   }
 }
 
 template <typename I>
 __global__ __launch_bounds__(kBlock) void k_set_basis0(double2* __restrict__ a, I len) {
-  const I base = (I)blockIdx.x * (I)(kBlock * kItems) + (I)threadIdx.x;
+  FOR_EACH_BLOCK(len) {
+    const I base = blk_ * (I)(kBlock * kItems) + (I)threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < kItems; ++j) {
-    const I i = base + (I)(j * kBlock);
-    if (i < len) a[i] = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+    for (int j = 0; j < kItems; ++j) {
+      const I i = base + (I)(j * kBlock);
+      if (i < len) a[i] = make_double2(i == 0 ? 1.0 : 0.0, 0.0);
+    }
   }
 }
 
 // ---- host helpers -----------------------------------------------------------
+constexpr uint64_t kMaxGrid = 1ULL << 20;  // 2^28 work-items per dispatch at most
 inline unsigned grid_for(uint64_t count) {
   uint64_t g = (count + (uint64_t)(kBlock * kItems) - 1) / (uint64_t)(kBlock * kItems);
+  if (g > kMaxGrid) g = kMaxGrid;
   return (unsigned)(g ? g : 1);
 }
 
