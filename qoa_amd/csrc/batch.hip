@@ -70,7 +70,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_block_batch(float2* __restric
                                                                const float* __restrict__ w,
                                                                const BlkItem* __restrict__ items,
                                                                int nitems) {
-  __shared__ float sw[kWFloats];
   int lo = 0, hi = nitems - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -82,7 +81,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_block_batch(float2* __restric
   ba.ncols = it.ncols;
 #pragma unroll
   for (int j = 0; j < kBlockQ; ++j) ba.q[j] = it.q[j];
-  block_apply<uint32_t>(states + it.state_off, w + it.w_off, sw, ba, blockIdx.x - it.wg_begin);
+  block_apply<uint32_t>(states + it.state_off, w + it.w_off, ba, blockIdx.x - it.wg_begin);
 }
 
 // |0...0> in every C64 instance: zero everything, then 1 at each start.

@@ -47,12 +47,13 @@ __device__ __forceinline__ I rowoff(uint32_t r, const BlockArgs& ba) {
 }
 
 // One workgroup's share of a block application: stages the block's A
-// operands w in LDS (sw, kWFloats floats), then its waves multiply columns
+// operands w in LDS (kWFloats floats), then its waves multiply columns
 // wg * 512 .. wg * 512 + 511 of the state a (32 x ncols view). Every thread
 // of the workgroup calls it (it has a barrier).
 template <typename I>
 __device__ __forceinline__ void block_apply(float2* __restrict__ a, const float* __restrict__ w,
-                                            float* sw, const BlockArgs& ba, uint64_t wg) {
+                                            const BlockArgs& ba, uint64_t wg) {
+  __shared__ float sw[kWFloats];
   for (int i = threadIdx.x; i < kWFloats / 4; i += kBlockThreads)
     reinterpret_cast<float4*>(sw)[i] = reinterpret_cast<const float4*>(w)[i];
   __syncthreads();
@@ -76,6 +77,11 @@ __device__ __forceinline__ void block_apply(float2* __restrict__ a, const float*
   bool ok, nok = false;
   load_group(0, cur, base, ok);
   for (int g = 0; g < kColGroups; ++g) {
+    // The A operands are read from LDS again for every column group: hoisted
+    // out of this loop they take 64 more VGPRs per lane (190 instead of 130,
+    // two waves per SIMD instead of three), and the block took 3821 instead
+    // of 3585 us at 30 qubits.
+    asm volatile("" ::: "memory");
     const bool more = g + 1 < kColGroups && wave_col0 + (uint64_t)(g + 1) * 32 < ba.ncols;
     if (more) load_group(g + 1, nxt, nbase, nok);
     f32x16 acc0 = {}, acc1 = {};

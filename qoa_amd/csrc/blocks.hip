@@ -30,8 +30,7 @@ template <typename I>
 __global__ __launch_bounds__(kBlockThreads) void k_block5(float2* __restrict__ a,
                                                          const float* __restrict__ w,
                                                          BlockArgs ba) {
-  __shared__ float sw[kWFloats];
-  block_apply<I>(a, w, sw, ba, blockIdx.x);
+  block_apply<I>(a, w, ba, blockIdx.x);
 }
 
 // Per-block partial sums of |a|^2 (fixed grid, fixed order).
