@@ -31,7 +31,7 @@ struct WorkItem {
   uint32_t traj;
   uint32_t frame_in, frame_out;  // slots of the X frame before / after the pass
   uint8_t k, init, noise, pad0;
-  uint8_t spos[16];
+  uint32_t spw[4];  // tile qubit positions as in Geo (whole words: scalar loads)
   uint8_t pad1[20];
 };
 static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
@@ -63,10 +63,10 @@ void k_batch_reg(double2* __restrict__ states,
   Geo geo;
   geo.k = it.k;
 #pragma unroll
-  for (int j = 0; j < 16; ++j) geo.spos[j] = it.spos[j];
+  for (int j = 0; j < 4; ++j) geo.spw[j] = it.spw[j];
   I base = (I)(blockIdx.x - it.block_begin);
   for (int j = 0; j < geo.k; ++j) {
-    const int b = geo.spos[j];
+    const int b = (int)spos_at(geo, j);
     base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
   }
   // Groups of 11+-qubit tiles (the only ones the swap-free kernel runs) have

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kThreads) void k_fused(double2* __restrict__ a,
   // tile base: the block index spread over the bits outside S
   I base = (I)blockIdx.x;
   for (int j = 0; j < geo.k; ++j) {
-    const int b = geo.spos[j];
+    const int b = (int)spos_at(geo, j);
     base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
   }
   for (uint32_t l = tid; l < T; l += kThreads) lds[l] = a[base | scatter<I>(l, geo)];
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5))) void k
   extern __shared__ double2 lds[];
   I base = (I)blockIdx.x;
   for (int j = 0; j < geo.k; ++j) {
-    const int b = geo.spos[j];
+    const int b = (int)spos_at(geo, j);
     base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
   }
   reg_tile<I, F, false, false, false>(a, base, geo, ops, nops, pool, lds, true, false, NoiseArgs{});
@@ -250,7 +250,7 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
   for (const PassPlan& pp : P.passes) {
     Geo geo{};
     geo.k = (int)k;
-    std::memcpy(geo.spos, pp.spos, sizeof(geo.spos));
+    std::memcpy(geo.spw, pp.spos, sizeof(geo.spw));
     const int nops = (int)(pp.op_end - pp.op_begin);
     const TileOp* ops = c->d_ops + pp.op_begin;
     double2* a = (double2*)s->amps;

@@ -331,7 +331,7 @@ int launch_step(qoa_server* s) {
       w.nops = pp.op_end - pp.op_begin;
       w.k = (uint8_t)j->plan.k;
       w.init = r == 0 ? 1 : 0;
-      std::memcpy(w.spos, pp.spos, 16);
+      std::memcpy(w.spw, pp.spos, 16);
       built.push_back(w);
       nblk.push_back(len >> j->plan.k);
       ++nitems;
