@@ -747,8 +747,11 @@ void merge_diag_runs(std::vector<TileOp>* ops, size_t begin, std::vector<double>
       m.aux = (uint32_t)pool->size();
       m.lmask = (uint32_t)terms.size();  // the term count
       m.rt = 0xFF;
+      for (size_t t = 0; t < terms.size(); ++t)
+        if (t + 1 == terms.size() || terms[t + 1].rs != terms[t].rs) terms[t].rs |= kDiagGroupEnd;
+      // the terms, then one spare zero term: the kernel reads a term ahead
       const size_t at = pool->size();
-      pool->resize(at + kDiagTermDoubles * terms.size());
+      pool->resize(at + kDiagTermDoubles * (terms.size() + 1), 0.0);
       std::memcpy(pool->data() + at, terms.data(), sizeof(DiagTerm) * terms.size());
       out.push_back(m);
     }

@@ -29,7 +29,9 @@ enum TileKind : uint8_t {
 // tl, and the register index holds the slots rs. Terms come grouped by rs
 // (ascending); a group's factors are multiplied together (in term order)
 // per thread, then into the registers whose index holds rs. 48 bytes = 6
-// pool doubles.
+// pool doubles. The last term of a group carries kDiagGroupEnd in rs; a
+// TK_DIAGN op's terms are followed by one spare zero term.
+constexpr uint32_t kDiagGroupEnd = 1u << 31;
 struct DiagTerm {
   uint32_t rs, tl;
   uint64_t gmask, gval;

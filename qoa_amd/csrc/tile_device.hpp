@@ -1352,22 +1352,30 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const
       break;
     case TK_DIAGN: {  // merged diagonal run: DiagTerm records, grouped by register slots
       // (tolerance mode: only the FMA instantiation carries this code, so the
-      // exact kernels keep their register allocation)
+      // exact kernels keep their register allocation). A term's words are
+      // loaded while the term before it is applied (every term list ends
+      // with a spare term), one scalar load wait per term.
       if constexpr (!F) break;
       const uint32_t* tw = reinterpret_cast<const uint32_t*>(pool + op.aux());
       const uint32_t nt = op.w[9];
       constexpr uint32_t W = kDiagTermDoubles * 2;  // words per term
-      uint32_t i = 0;
-      while (i < nt) {
-        const uint32_t rs = tw[W * i];
-        double2 acc = make_double2(1.0, 0.0);
-        for (; i < nt && tw[W * i] == rs; ++i) {
-          const uint32_t* w = tw + W * i;
-          const uint64_t gm = w[2] | (uint64_t)w[3] << 32, gv = w[4] | (uint64_t)w[5] << 32;
-          if ((tkey & gm) != gv) continue;  // uniform
-          const double cr = __builtin_bit_cast(double, w[6] | (uint64_t)w[7] << 32);
-          const double ci = __builtin_bit_cast(double, w[8] | (uint64_t)w[9] << 32);
-          const uint32_t ttl = w[1];
+      constexpr int kTW = 10;                       // words read: rs, tl, gmask, gval, c
+      uint32_t cur[kTW], nxt[kTW];
+#pragma unroll
+      for (int j = 0; j < kTW; ++j) cur[j] = tw[j];
+      double2 acc = make_double2(1.0, 0.0);
+      for (uint32_t i = 0; i < nt; ++i) {
+        const uint32_t* wn = tw + W * (i + 1);
+#pragma unroll
+        for (int j = 0; j < kTW; ++j) nxt[j] = wn[j];
+        // a compiler memory barrier keeps the loads here (they would sink to
+        // the end of the iteration, next to their first use)
+        asm volatile("" ::: "memory");
+        const uint64_t gm = cur[2] | (uint64_t)cur[3] << 32, gv = cur[4] | (uint64_t)cur[5] << 32;
+        if ((tkey & gm) == gv) {  // uniform
+          const double cr = __builtin_bit_cast(double, cur[6] | (uint64_t)cur[7] << 32);
+          const double ci = __builtin_bit_cast(double, cur[8] | (uint64_t)cur[9] << 32);
+          const uint32_t ttl = cur[1];
           if (ttl == 0) {
             ip_mulc<F, true>(acc, make_double2(cr, ci));
           } else {
@@ -1375,7 +1383,12 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const
             ip_mulc<F>(acc, make_double2(on ? cr : 1.0, on ? ci : 0.0));
           }
         }
-        reg_phase_rs<F, false>(v, acc, rs);
+        if (cur[0] & kDiagGroupEnd) {  // the group of slots cur[0] & 15 ends
+          reg_phase_rs<F, false>(v, acc, cur[0] & 15u);
+          acc = make_double2(1.0, 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < kTW; ++j) cur[j] = nxt[j];
       }
     } break;
     default:

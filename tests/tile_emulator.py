@@ -148,11 +148,17 @@ def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int, pool=None) -> Non
                     cnt, at = int(op["lmask"]), int(op["aux"])
                     terms = pool[at:at + 6 * cnt].view(DIAG_TERM)
                     assert len(terms) == cnt
-                    assert np.all(np.diff(terms["rs"].astype(np.int64)) >= 0), "terms grouped"
+                    slots = terms["rs"] & 0xF
+                    assert np.all(np.diff(slots.astype(np.int64)) >= 0), "terms grouped"
+                    # the group-end flag (bit 31) marks exactly each group's last term
+                    ends = (terms["rs"] >> 31).astype(bool)
+                    assert np.array_equal(ends, np.append(np.diff(slots) != 0, True)), "group ends"
+                    spare = pool[at + 6 * cnt:at + 6 * cnt + 6]
+                    assert len(spare) == 6 and not spare.any(), "spare term after the terms"
                     R = sum(1 << b for b in layout)
-                    for rs in np.unique(terms["rs"]):
+                    for rs in np.unique(slots):
                         ar, ai = np.ones(T), np.zeros(T)
-                        for t in terms[terms["rs"] == rs]:
+                        for t in terms[slots == rs]:
                             if (base & int(t["gmask"])) != int(t["gval"]):
                                 continue
                             tl = int(t["tl"])
