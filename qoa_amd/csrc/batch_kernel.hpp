@@ -117,7 +117,7 @@ inline void group_items(const std::vector<WorkItem>& in, const std::vector<uint6
 }
 
 // Lets every instantiation take dynamic LDS beyond the default limit (the
-// exchange buffer holds 2^(k - reg_xchg_split(k)) amplitudes).
+// exchange buffer holds 2^(k - reg_xchg_split(k, true)) amplitudes).
 inline void batch_kernel_attrs() {
   const void* fns[3] = {(const void*)k_batch_reg<uint32_t, false, false>,
                         (const void*)k_batch_reg<uint32_t, false, true>,
@@ -135,7 +135,7 @@ inline void batch_kernel_attrs() {
 inline void launch_group(hipStream_t st, const ItemGroup& g, double2* states,
                          const WorkItem* d_items, const TileOp* ops, const double* pool,
                          uint64_t* frames, const uint32_t* codes) {
-  const size_t lbytes = sizeof(double2) * reg_xchg_elems(g.kmax);
+  const size_t lbytes = sizeof(double2) * reg_xchg_elems(g.kmax, true);
   auto fn = g.noise ? k_batch_reg<uint32_t, true, true>
                     : (g.kmax <= 10 ? k_batch_reg<uint32_t, false, true>
                                     : k_batch_reg<uint32_t, false, false>);

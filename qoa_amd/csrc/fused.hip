@@ -257,9 +257,9 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
     timing_begin(s, KK_FUSED, 32.0 * (double)s->len);
     if (pp.reg_mode) {
       // 16 amplitudes per thread; LDS holds the exchange buffer for layout
-      // changes (half a 12-qubit tile)
+      // changes (a quarter of a 12-qubit tile)
       const unsigned threads = 1u << (k - kRegBits);
-      const size_t lbytes = sizeof(double2) * reg_xchg_elems(k);
+      const size_t lbytes = sizeof(double2) * reg_xchg_elems(k, false);
       // merged diagonal runs exist in the FMA instantiation only
       const bool fma = (opt.flags & (QOA_FUSE_FMA | QOA_FUSE_MERGE_DIAG)) != 0;
       const dim3 grid((unsigned)tiles), block(threads);

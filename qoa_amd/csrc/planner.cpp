@@ -302,7 +302,7 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
 // Each thread of the register-resident kernel holds the kRegAmps amplitudes
 // whose local indices differ in the register bits R. Ops that mix only bits
 // of R (and every diagonal op) run in registers; moving to a new R costs
-// TK_LAYOUT ops (LDS exchanges in 2^reg_xchg_split(k) rounds, or swap
+// TK_LAYOUT ops (LDS exchanges in 2^reg_xchg_split(k, batched) rounds, or swap
 // lists). The same greedy list scheduling as the pass planner picks R: ops
 // on disjoint bits commute, so the emitted order stays a valid reordering. The pass starts in the layout that holds the top
 // kRegBits local bits in registers (coalesced HBM access) and ends in one
@@ -480,7 +480,7 @@ inline bool use_swaps(uint32_t k, bool swaps) { return swaps && k <= 10; }
 
 // Whole-tile exchange op from layout `from` to `to`: ins = the new slot
 // bits, t < t2 = slots whose bit is the same in both (the exchange of a 2^k
-// tile runs in 2^reg_xchg_split(k) rounds keyed by those slots' bits).
+// tile runs in 2^reg_xchg_split(k, batched) rounds keyed by those slots' bits).
 TileOp exchange_op(const RegSlots& from, const RegSlots& to) {
   TileOp op{};
   op.kind = TK_LAYOUT;
@@ -496,9 +496,10 @@ TileOp exchange_op(const RegSlots& from, const RegSlots& to) {
 
 // Whole-tile exchange to the register set nr (bits in both keep their
 // slots); the thread index bits then hold the other local bits in ascending
-// order. One op changes at most kRegBits - reg_xchg_split(k) slots.
-void exchange_layout(RegSlots* cur, uint32_t nr, uint32_t k, std::vector<TileOp>* out) {
-  const int max_changes = kRegBits - (int)reg_xchg_split(k);
+// order. One op changes at most kRegBits - reg_xchg_split(k, batched) slots.
+void exchange_layout(RegSlots* cur, uint32_t nr, uint32_t k, bool batched,
+                     std::vector<TileOp>* out) {
+  const int max_changes = kRegBits - (int)reg_xchg_split(k, batched);
   while (cur->set != nr) {
     RegSlots to = *cur;
     uint32_t in = nr & ~cur->set;
@@ -525,7 +526,7 @@ void exchange_layout(RegSlots* cur, uint32_t nr, uint32_t k, std::vector<TileOp>
 void move_layout(RegSlots* cur, uint32_t nr, uint32_t k, bool swaps, std::vector<TileOp>* out) {
   if (cur->set == nr) return;
   if (!use_swaps(k, swaps)) {
-    exchange_layout(cur, nr, k, out);
+    exchange_layout(cur, nr, k, swaps, out);
     return;
   }
   SwapList sl;
@@ -574,7 +575,7 @@ void store_layout(RegSlots* cur, uint32_t k, bool swaps, std::vector<TileOp>* ou
       uint32_t target = cur->set & ~low;
       for (int b = (int)k - 1; b >= 0 && __builtin_popcount(target) < kRegBits; --b)
         if (!(low & (1u << b))) target |= 1u << b;
-      exchange_layout(cur, target, k, out);
+      exchange_layout(cur, target, k, swaps, out);
     }
     return;
   }
@@ -621,7 +622,8 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
   cur.nt = k - kRegBits;
   for (uint32_t m = 0; m < cur.nt; ++m) cur.tb[m] = (uint8_t)m;
   uint32_t R = L0;
-  const int keep = use_swaps(k, swaps) ? 0 : (int)reg_xchg_split(k);  // slots an exchange keeps
+  // slots an exchange keeps (plans with swap lists are the batched kernels')
+  const int keep = use_swaps(k, swaps) ? 0 : (int)reg_xchg_split(k, swaps);
   uint32_t last_gi = 0xFFFFFFFFu;
   while (!rem.empty()) {
     const int ready = seg_sim(m, rem, R, allb);

@@ -52,11 +52,21 @@ constexpr uint8_t kM2General = 0, kM2Real = 1, kM2Rx = 2;
 constexpr int kRegBits = 4;
 constexpr int kRegAmps = 1 << kRegBits;
 // Register passes exchange amplitudes through one LDS buffer of 2^(k-q)
-// elements, q = reg_xchg_split(k) (the exchange then takes 2^q rounds, and
-// an exchange changes at most kRegBits - q register slots): the buffer holds
-// at most 2048 elements (32 KiB).
-constexpr uint32_t reg_xchg_split(uint32_t k) { return k <= 11 ? 0u : k == 12 ? 1u : 2u; }
-constexpr uint32_t reg_xchg_elems(uint32_t k) { return 1u << (k - reg_xchg_split(k)); }
+// elements, q = reg_xchg_split(k, batched) (the exchange then takes 2^q
+// rounds, and an exchange changes at most kRegBits - q register slots): the
+// buffer holds at most 2048 elements (32 KiB). At k = 12 the fused kernel
+// takes quarter-tile rounds (16 KiB): it holds 96 VGPRs, five waves per
+// SIMD, and five 256-thread blocks fit a CU's LDS with 16 KiB buffers but
+// not with 32 KiB ones (measured: four blocks, 3.6 % slower on the 28-qubit
+// layered circuit despite fewer exchanges). The batched kernels hold more
+// VGPRs (at most four waves per SIMD), so their occupancy does not depend on
+// the buffer and they keep half-tile rounds, which change one more slot.
+constexpr uint32_t reg_xchg_split(uint32_t k, bool batched) {
+  return k <= 11 ? 0u : (k == 12 && batched) ? 1u : 2u;
+}
+constexpr uint32_t reg_xchg_elems(uint32_t k, bool batched) {
+  return 1u << (k - reg_xchg_split(k, batched));
+}
 // PlanOptions.flags bit (internal): register mode for every tile of at least
 // kRegBits qubits (the batched kernel runs small whole-state tiles this way);
 // the default tile stays at 12 qubits for states of 13..17 qubits too.

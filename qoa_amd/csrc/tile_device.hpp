@@ -841,7 +841,7 @@ __device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u) {
 // ---- exchanges through LDS --------------------------------------------------
 // One buffer of 2^(k-Q) amplitudes, indexed through swz (bits 4..7 of an
 // index XORed into bits 0..3, which spreads the strided layouts over the
-// banks without padding). Q = reg_xchg_split(k) register slots (A, then B),
+// banks without padding). Q = reg_xchg_split(k, batched) register slots (A, then B),
 // whose bits are the same before and after the exchange, split it into 2^Q
 // rounds: round c moves the amplitudes whose bits at those slots read c, and
 // a buffer index is the local index without those bits. A round waits at a
@@ -931,12 +931,11 @@ __device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const
   }
 }
 
-// Exchange keyed by the first reg_xchg_split(k) of slots a < b (unchanged by
-// it); 2^reg_xchg_split(k) rounds.
+// Exchange keyed by the first q of slots a < b (unchanged by it); 2^q rounds,
+// q = reg_xchg_split(k, batched).
 __device__ __forceinline__ void xchg_any(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
-                                         uint32_t tw, uint32_t tr, uint32_t a, uint32_t b, int k,
-                                         Xchg& X, bool act) {
-  const uint32_t q = reg_xchg_split((uint32_t)k);
+                                         uint32_t tw, uint32_t tr, uint32_t a, uint32_t b,
+                                         uint32_t q, Xchg& X, bool act) {
   if (q == 0) {
     xchg<0, 0, 1>(v, L, N, tw, tr, X, act);
   } else if (q == 1) {
@@ -961,8 +960,8 @@ __device__ __forceinline__ void xchg_any(double2 (&v)[kR], const RegLayout& L, c
 // Whole-tile layout change L -> N (N's thread bits in ascending order); a < b
 // = slots holding the same bit in both.
 __device__ __forceinline__ void relayout(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
-                                         uint32_t a, uint32_t b, int k, Xchg& X, bool act) {
-  xchg_any(v, L, N, L.thr_l, N.thr_l, a, b, k, X, act);
+                                         uint32_t a, uint32_t b, uint32_t q, Xchg& X, bool act) {
+  xchg_any(v, L, N, L.thr_l, N.thr_l, a, b, q, X, act);
 }
 
 // ---- layout swaps ------------------------------------------------------------
@@ -1156,6 +1155,7 @@ __device__ __forceinline__ void reg_flip(double2 (&v)[kR]);
 
 // X flips of the local bits in xl: register bits by exchanging registers,
 // thread bits through LDS (thread t reads the amplitudes of thr_l ^ x).
+// Trajectory noise runs in the batched kernels only: their exchange split.
 __device__ __forceinline__ void flip_bits(double2 (&v)[kR], const RegLayout& L, uint32_t xl,
                                           int k, Xchg& X, bool act) {
   uint32_t rmask = 0;
@@ -1166,7 +1166,7 @@ __device__ __forceinline__ void flip_bits(double2 (&v)[kR], const RegLayout& L, 
   if (xl & L.rb[2]) reg_flip<2, false>(v);
   if (xl & L.rb[3]) reg_flip<3, false>(v);
   const uint32_t xt = xl & ~rmask;
-  if (xt) xchg_any(v, L, L, L.thr_l, L.thr_l ^ xt, 2, 3, k, X, act);
+  if (xt) xchg_any(v, L, L, L.thr_l, L.thr_l ^ xt, 2, 3, reg_xchg_split((uint32_t)k, true), X, act);
 }
 
 // Trajectory noise state of one tile pass (on == 0: no noise).
@@ -1495,7 +1495,9 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
       } else {  // whole-tile exchange; two unchanged slots in t, t2
         RegLayout N;
         set_layout(N, tid, k, rpos);
-        relayout(v, L, N, (op.w[0] >> 8) & 0xFFu, (op.w[0] >> 16) & 0xFFu, k, X, act);
+        // batched kernels test idle waves per wave (SCALAR_IDLE): their split
+        relayout(v, L, N, (op.w[0] >> 8) & 0xFFu, (op.w[0] >> 16) & 0xFFu,
+                 reg_xchg_split((uint32_t)k, SCALAR_IDLE), X, act);
         L = N;
         if (SWAPS) {
           uint32_t rmask = 0;

@@ -1,5 +1,6 @@
 """Index model of the register kernels' LDS exchanges (CPU): the 2^q rounds
-(q = 0, 1, 2 by tile size) of a layout change and of thread-bit X flips
+(q = 0, 1, 2 by tile size; at 12 qubits 2 in the fused kernel and 1 in the
+batched kernels) of a layout change and of thread-bit X flips
 through one buffer,
 with the buffer indexing written as the device code computes it (squeeze,
 the XOR swizzle, per-thread part XOR per-register part). Every write lands inside its buffer, no two
@@ -26,12 +27,12 @@ def squeeze(l, bits):
     return l
 
 
-def split(k):
-    return 0 if k <= 11 else 1 if k == 12 else 2
+def split(k, batched):
+    return 0 if k <= 11 else 1 if (k == 12 and batched) else 2
 
 
-def stride(k):
-    return 1 << (k - split(k))
+def stride(k, batched):
+    return 1 << (k - split(k, batched))
 
 
 def set_layout(tid, k, rpos):
@@ -49,11 +50,11 @@ def regl(rpos, r):
     return sum(1 << rpos[j] for j in range(K_REG) if r >> j & 1)
 
 
-def xchg(vals, k, rold, rnew, tw_of, tr_of, A, B):
+def xchg(vals, k, rold, rnew, tw_of, tr_of, A, B, batched):
     """vals[t][r]: register r of thread t. Returns the registers after the
     exchange; tw_of / tr_of give a thread's write / read thread part."""
     T = 1 << (k - K_REG)
-    Q = split(k)
+    Q = split(k, batched)
     sl = [A, B][:Q]
     bits = [rold[j] for j in sl]
     assert all(rnew[j] == rold[j] for j in sl)
@@ -69,7 +70,7 @@ def xchg(vals, k, rold, rnew, tw_of, tr_of, A, B):
                 x ^= o[j]
         return x
     rnd = lambda r: sum(((r >> j) & 1) << i for i, j in enumerate(sl))
-    S = stride(k)
+    S = stride(k, batched)
     assert S <= 2048  # 32 KiB
     out = [list(v) for v in vals]
     for c in range(1 << Q):
@@ -102,8 +103,9 @@ def tile_regs(k, rpos):
     return [[set_layout(t, k, rpos) | regl(rpos, r) for r in range(1 << K_REG)] for t in range(T)]
 
 
+@pytest.mark.parametrize("batched", [False, True])
 @pytest.mark.parametrize("k", [5, 6, 8, 10, 11, 12, 13])
-def test_layout_change_rounds(k):
+def test_layout_change_rounds(k, batched):
     rng = np.random.default_rng(k)
     for _ in range(40 if k < 11 else 8):
         rold = [int(x) for x in rng.choice(k, K_REG, replace=False)]
@@ -115,7 +117,7 @@ def test_layout_change_rounds(k):
                 rnew[j] = free.pop(int(rng.integers(len(free))))
         vals = tile_regs(k, rold)
         out = xchg(vals, k, rold, rnew, lambda t: set_layout(t, k, rold),
-                   lambda t: set_layout(t, k, rnew), A, B)
+                   lambda t: set_layout(t, k, rnew), A, B, batched)
         assert out == tile_regs(k, rnew)
 
 
@@ -128,7 +130,7 @@ def test_thread_bit_flip_rounds(k):
         xt = sum(1 << b for b in tbits if rng.random() < 0.4) or (1 << tbits[0])
         vals = tile_regs(k, rpos)
         out = xchg(vals, k, rpos, rpos, lambda t: set_layout(t, k, rpos),
-                   lambda t: set_layout(t, k, rpos) ^ xt, 2, 3)
+                   lambda t: set_layout(t, k, rpos) ^ xt, 2, 3, True)  # noise: batched only
         want = [[v ^ xt for v in row] for row in vals]
         assert out == want
 

@@ -89,9 +89,12 @@ def _check_reg_fields(op, kind: int, lm: int, layout) -> None:
         assert int(op["rt"]) == want, "RZ slot"
 
 
-def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int, pool=None) -> None:
+def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int, pool=None,
+                batched: bool = False) -> None:
     """Applies the program to the complex128 state `a` in place (`pool`: the
-    plan's coefficient pool, needed for merged diagonal ops)."""
+    plan's coefficient pool, needed for merged diagonal ops; `batched`: a
+    plan for the batched kernels, whose 12-qubit exchanges keep one slot
+    instead of two)."""
     re, im = a.real.copy(), a.imag.copy()
     T = 1 << k
     loc = np.arange(T)
@@ -114,7 +117,7 @@ def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int, pool=None) -> Non
                     if not sw >> 31:
                         # whole-tile exchange: the first q of slots (t, t2)
                         # keep their bit (the rounds are keyed by those bits)
-                        q = 0 if k <= 11 else 1 if k == 12 else 2
+                        q = 0 if k <= 11 else 1 if (k == 12 and batched) else 2
                         for ks in [int(op["t"]), int(op["t2"])][:q]:
                             assert layout[ks] == new[ks], "kept slots"
                         assert sorted(new) == sorted(set(new)), "slots hold distinct bits"
