@@ -478,10 +478,24 @@ void swap_rr(RegSlots* L, uint32_t i, uint32_t j, SwapList* sl, std::vector<Tile
 // there costs 52-155 us at 28 qubits against 124 us for a whole exchange).
 inline bool use_swaps(uint32_t k, bool swaps) { return swaps && k <= 10; }
 
+// Exchange buffer byte offset of local index l (the kernels' squeeze and
+// swz): l without the bits at the q split positions pa (and pb), bits 4..7
+// XORed into bits 0..3, times 16 bytes.
+uint32_t xchg_offset(uint32_t l, uint32_t q, uint32_t pa, uint32_t pb) {
+  auto squeeze1 = [](uint32_t x, uint32_t p) { return ((x >> (p + 1)) << p) | (x & ((1u << p) - 1u)); };
+  if (q == 1) l = squeeze1(l, pa);
+  if (q == 2) l = squeeze1(squeeze1(l, pa > pb ? pa : pb), pa > pb ? pb : pa);
+  return (l ^ ((l >> 4) & 15u)) << 4;
+}
+
 // Whole-tile exchange op from layout `from` to `to`: ins = the new slot
 // bits, t < t2 = slots whose bit is the same in both (the exchange of a 2^k
-// tile runs in 2^reg_xchg_split(k, batched) rounds keyed by those slots' bits).
-TileOp exchange_op(const RegSlots& from, const RegSlots& to) {
+// tile runs in 2^q rounds keyed by the first q of those slots' bits,
+// q = reg_xchg_split(k, batched)). The coefficient words carry what the
+// kernel would otherwise derive per exchange (kXchgSorted, kXchgOld,
+// kXchgNew): the new register bit positions in ascending order and every
+// slot's buffer offset under both layouts.
+TileOp exchange_op(const RegSlots& from, const RegSlots& to, uint32_t q) {
   TileOp op{};
   op.kind = TK_LAYOUT;
   int kept[kRegBits], nk = 0;
@@ -491,6 +505,18 @@ TileOp exchange_op(const RegSlots& from, const RegSlots& to) {
   }
   op.t = (uint8_t)(nk > 0 ? kept[0] : 0);
   op.t2 = (uint8_t)(nk > 1 ? kept[1] : op.t + 1);
+  uint32_t w[kXchgWords] = {};
+  uint8_t sorted[kRegBits];
+  for (int j = 0; j < kRegBits; ++j) sorted[j] = to.s[j];
+  std::sort(sorted, sorted + kRegBits);
+  for (int j = 0; j < kRegBits; ++j) w[kXchgSorted] |= (uint32_t)sorted[j] << (8 * j);
+  const uint32_t pa = from.s[op.t], pb = from.s[op.t2 < kRegBits ? op.t2 : op.t];
+  for (int j = 0; j < kRegBits; ++j) {
+    const bool split = (q >= 1 && j == op.t) || (q == 2 && j == op.t2);
+    w[kXchgOld + j] = split ? 0u : xchg_offset(1u << from.s[j], q, pa, pb);
+    w[kXchgNew + j] = split ? 0u : xchg_offset(1u << to.s[j], q, pa, pb);
+  }
+  std::memcpy(op.c, w, sizeof(w));
   return op;
 }
 
@@ -512,7 +538,7 @@ void exchange_layout(RegSlots* cur, uint32_t nr, uint32_t k, bool batched,
       to.s[j] = (uint8_t)b;
       ++changed;
     }
-    out->push_back(exchange_op(*cur, to));
+    out->push_back(exchange_op(*cur, to, reg_xchg_split(k, batched)));
     uint32_t m = 0;
     for (uint32_t b = 0; b < k; ++b)
       if (!(to.set & (1u << b))) to.tb[m++] = (uint8_t)b;
@@ -544,8 +570,9 @@ void move_layout(RegSlots* cur, uint32_t nr, uint32_t k, bool swaps, std::vector
 // Puts register bits lo and hi into slots 2 and 3 (where the kernels apply
 // 4x4 ops) by slot swaps: one swap list when the kernel takes swap lists,
 // else one whole-tile exchange per swap (two slots stay put; the thread
-// bits do not change).
-void place_pair(RegSlots* cur, uint32_t lo, uint32_t hi, bool swaps, std::vector<TileOp>* out) {
+// bits do not change; q: the exchange split).
+void place_pair(RegSlots* cur, uint32_t lo, uint32_t hi, bool swaps, uint32_t q,
+                std::vector<TileOp>* out) {
   SwapList sl;
   const uint32_t want[2] = {lo, hi};
   for (int d = 0; d < 2; ++d) {
@@ -557,7 +584,7 @@ void place_pair(RegSlots* cur, uint32_t lo, uint32_t hi, bool swaps, std::vector
     } else {
       RegSlots to = *cur;
       std::swap(to.s[src], to.s[dst]);
-      out->push_back(exchange_op(*cur, to));
+      out->push_back(exchange_op(*cur, to, q));
       *cur = to;
     }
   }
@@ -641,7 +668,7 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
         TileOp op = ops[i];
         if (op.kind == TK_MAT4) {  // the lower local bit in slot 2, the higher in slot 3
           place_pair(&cur, op.t < op.t2 ? op.t : op.t2, op.t < op.t2 ? op.t2 : op.t,
-                     use_swaps(k, swaps), out);
+                     use_swaps(k, swaps), reg_xchg_split(k, swaps), out);
           op.ins[0] = 2;
           op.ins[1] = 3;
         } else if (m[i].act) {

@@ -19,7 +19,8 @@ enum TileKind : uint8_t {
   TK_SWAP = 7,    // exchange (l|m2, l|m1) for l with both bits clear (and lmask set)
   TK_MAT4 = 8,    // 4x4 on local (t, t2) = (q1, q2); coefficients at pool[aux]
   TK_LAYOUT = 9,  // register passes: ins[0..3] = local bits of register slots 0..3;
-                  // slots t < t2 keep their bit where the exchange needs it
+                  // slots t < t2 keep their bit where the exchange needs it;
+                  // a whole-tile exchange carries kXchgWords words in c
   TK_DIAGN = 10   // register passes (QOA_FUSE_MERGE_DIAG): a run of diagonal ops as
                   // lmask terms at pool[aux] (DiagTerm), grouped by register slots
 };
@@ -40,6 +41,12 @@ struct DiagTerm {
 };
 static_assert(sizeof(DiagTerm) == 48, "DiagTerm layout");
 constexpr uint32_t kDiagTermDoubles = sizeof(DiagTerm) / sizeof(double);
+
+// A whole-tile exchange op's words in c (as 32-bit words): the new register
+// bit positions in ascending order (a byte each), then the exchange buffer's
+// byte offset of every register slot's bit under the old layout and under the
+// new one (0 for the slots that key the rounds).
+constexpr int kXchgSorted = 0, kXchgOld = 1, kXchgNew = 5, kXchgWords = 9;
 
 // TK_M shapes (t2): general, real, or real diagonal with imaginary off-diagonal.
 constexpr uint8_t kM2General = 0, kM2Real = 1, kM2Rx = 2;

@@ -134,6 +134,22 @@ __device__ __forceinline__ uint32_t regl(const RegLayout& L, int r) {
   return o;
 }
 
+// set_layout for a layout whose register bit positions are also given in
+// ascending order (a byte each in `sorted`, as exchange ops carry them): the
+// thread's local bits are tid with a zero inserted at each of them.
+__device__ __forceinline__ void set_layout_sorted(RegLayout& L, uint32_t tid, const uint8_t* rpos,
+                                                  uint32_t sorted) {
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) L.rb[j] = 1u << rpos[j];
+  uint32_t l = tid;
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) {
+    const uint32_t p = (sorted >> (8 * j)) & 0xFFu;
+    l = ((l >> p) << (p + 1)) | (l & ((1u << p) - 1u));
+  }
+  L.thr_l = l;
+}
+
 __device__ __forceinline__ void set_layout(RegLayout& L, uint32_t tid, int k, const uint8_t* rpos) {
   uint32_t rmask = 0;
 #pragma unroll
@@ -902,14 +918,14 @@ __device__ __forceinline__ constexpr int round_of(int r) {
 
 // Register r of every thread moves from local index tw | regl(L, r) to
 // tr | regl(N, r): tw / tr are the thread parts (register bits clear) on the
-// writing and the reading side.
+// writing and the reading side; ow / orr the slot offsets (slot_offsets) of
+// the two layouts, pa / pb the positions of the slots A / B that key the
+// rounds.
 template <int Q, int A, int B>
-__device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
-                                     uint32_t tw, uint32_t tr, Xchg& X, bool act) {
-  const uint32_t pa = __builtin_ctz(L.rb[A]), pb = __builtin_ctz(L.rb[B]);
-  uint32_t ow[kRegBits], orr[kRegBits];
-  slot_offsets<Q, A, B>(L, pa, pb, ow);
-  slot_offsets<Q, A, B>(N, pa, pb, orr);
+__device__ __forceinline__ void xchg_off(double2 (&v)[kR], uint32_t pa, uint32_t pb,
+                                         const uint32_t (&ow)[kRegBits],
+                                         const uint32_t (&orr)[kRegBits], uint32_t tw,
+                                         uint32_t tr, Xchg& X, bool act) {
   const uint32_t bw = swz(squeeze<Q>(tw, pa, pb)) << 4, br = swz(squeeze<Q>(tr, pa, pb)) << 4;
   char* buf = reinterpret_cast<char*>(X.lds);
 #pragma unroll
@@ -929,6 +945,31 @@ __device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const
           v[r] = *reinterpret_cast<const double2*>(buf + (br ^ reg_offset<Q, A, B>(orr, r)));
     }
   }
+}
+
+template <int Q, int A, int B>
+__device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
+                                     uint32_t tw, uint32_t tr, Xchg& X, bool act) {
+  const uint32_t pa = __builtin_ctz(L.rb[A]), pb = __builtin_ctz(L.rb[B]);
+  uint32_t ow[kRegBits], orr[kRegBits];
+  slot_offsets<Q, A, B>(L, pa, pb, ow);
+  slot_offsets<Q, A, B>(N, pa, pb, orr);
+  xchg_off<Q, A, B>(v, pa, pb, ow, orr, tw, tr, X, act);
+}
+
+// xchg with the slot offsets the planner put in an exchange op (words w:
+// kXchgOld, kXchgNew of its coefficients).
+template <int Q, int A, int B>
+__device__ __forceinline__ void xchg_op(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
+                                        const uint32_t* w, Xchg& X, bool act) {
+  uint32_t ow[kRegBits], orr[kRegBits];
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) {
+    ow[j] = w[kXchgOld + j];
+    orr[j] = w[kXchgNew + j];
+  }
+  xchg_off<Q, A, B>(v, __builtin_ctz(L.rb[A]), __builtin_ctz(L.rb[B]), ow, orr, L.thr_l,
+                    N.thr_l, X, act);
 }
 
 // Exchange keyed by the first q of slots a < b (unchanged by it); 2^q rounds,
@@ -957,11 +998,31 @@ __device__ __forceinline__ void xchg_any(double2 (&v)[kR], const RegLayout& L, c
   }
 }
 
-// Whole-tile layout change L -> N (N's thread bits in ascending order); a < b
-// = slots holding the same bit in both.
+// Whole-tile layout change L -> N (N's thread bits in ascending order) of an
+// exchange op: a < b = slots holding the same bit in both, w = the op's
+// coefficient words (the planner's slot offsets).
 __device__ __forceinline__ void relayout(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
-                                         uint32_t a, uint32_t b, uint32_t q, Xchg& X, bool act) {
-  xchg_any(v, L, N, L.thr_l, N.thr_l, a, b, q, X, act);
+                                         uint32_t a, uint32_t b, uint32_t q, const uint32_t* w,
+                                         Xchg& X, bool act) {
+  if (q == 0) {
+    xchg_op<0, 0, 1>(v, L, N, w, X, act);
+  } else if (q == 1) {
+    switch (a) {  // B is not used for one split slot
+      case 0: xchg_op<1, 0, 1>(v, L, N, w, X, act); break;
+      case 1: xchg_op<1, 1, 2>(v, L, N, w, X, act); break;
+      case 2: xchg_op<1, 2, 3>(v, L, N, w, X, act); break;
+      default: xchg_op<1, 3, 0>(v, L, N, w, X, act); break;
+    }
+  } else {
+    switch (a * 4 + b) {
+      case 1: xchg_op<2, 0, 1>(v, L, N, w, X, act); break;
+      case 2: xchg_op<2, 0, 2>(v, L, N, w, X, act); break;
+      case 3: xchg_op<2, 0, 3>(v, L, N, w, X, act); break;
+      case 6: xchg_op<2, 1, 2>(v, L, N, w, X, act); break;
+      case 7: xchg_op<2, 1, 3>(v, L, N, w, X, act); break;
+      default: xchg_op<2, 2, 3>(v, L, N, w, X, act); break;
+    }
+  }
 }
 
 // ---- layout swaps ------------------------------------------------------------
@@ -1265,6 +1326,7 @@ __device__ __forceinline__ OpRegs load_op(const TileOp* __restrict__ p) {
 // words 0..7) and the rest (the 4x4 pool offset or swap list, noise fields,
 // coefficients), which only the work after dispatch reads.
 constexpr int kCoefFirst = 8, kCoefEnd = kOpWords;
+constexpr int kCoefWord = 12;  // c[0] as a 32-bit word of the op
 __device__ __forceinline__ bool head_word(int i) { return i < kCoefFirst; }
 __device__ __forceinline__ void load_head(OpRegs& r, const TileOp* __restrict__ p) {
   const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
@@ -1494,10 +1556,10 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
         apply_swaps(v, L, tbp, sw & 0x7FFFFFFFu, k, lds, act);
       } else {  // whole-tile exchange; two unchanged slots in t, t2
         RegLayout N;
-        set_layout(N, tid, k, rpos);
+        set_layout_sorted(N, tid, rpos, op.w[kCoefWord + kXchgSorted]);
         // batched kernels test idle waves per wave (SCALAR_IDLE): their split
         relayout(v, L, N, (op.w[0] >> 8) & 0xFFu, (op.w[0] >> 16) & 0xFFu,
-                 reg_xchg_split((uint32_t)k, SCALAR_IDLE), X, act);
+                 reg_xchg_split((uint32_t)k, SCALAR_IDLE), op.w + kCoefWord, X, act);
         L = N;
         if (SWAPS) {
           uint32_t rmask = 0;

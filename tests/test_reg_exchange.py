@@ -103,6 +103,18 @@ def tile_regs(k, rpos):
     return [[set_layout(t, k, rpos) | regl(rpos, r) for r in range(1 << K_REG)] for t in range(T)]
 
 
+def test_swizzle_is_a_linear_bijection():
+    """The buffer index map must be a bijection of [0, 2^m) for every buffer
+    size and linear over XOR (the kernels split an index into a per-thread
+    and a per-register part and XOR their images)."""
+    for m in range(1, 12):
+        img = [swz(x) for x in range(1 << m)]
+        assert sorted(img) == list(range(1 << m))
+    rng = np.random.default_rng(3)
+    for a, b in rng.integers(0, 1 << 11, size=(200, 2)):
+        assert swz(int(a) ^ int(b)) == swz(int(a)) ^ swz(int(b))
+
+
 @pytest.mark.parametrize("batched", [False, True])
 @pytest.mark.parametrize("k", [5, 6, 8, 10, 11, 12, 13])
 def test_layout_change_rounds(k, batched):
