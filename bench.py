@@ -45,10 +45,15 @@ FP32_MFMA_TFLOPS = 157.3
 FP64_TFLOPS = 78.6
 SIMDS = 1024          # 256 CUs x 4 SIMDs; a wave64 VALU or SALU instruction issues in 4 cycles
 CLOCK_MHZ = 2400.0    # peak engine clock
-QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG = 2, 4, 16
+QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_SPECIALIZE = 2, 4, 16, 64
 
 
 METHOD_NOTES = {
+    "fused-spec": "fused register passes (12-qubit tiles) as kernels compiled for this circuit "
+                  "(hipRTC, before the timed steps; extras.c2_specialize), FP64 with fused "
+                  "multiply-adds, runs of diagonal gates merged per register group: the "
+                  "fused-merge op-loop kernel's results bit for bit, within 1e-12 of the "
+                  "per-gate FP64 sweeps (accuracy field)",
     "fused-merge": "fused register passes (12-qubit tiles), FP64 with fused multiply-adds, "
                    "runs of diagonal gates merged per register group; within 1e-12 of the "
                    "per-gate FP64 sweeps (accuracy field)",
@@ -58,6 +63,13 @@ METHOD_NOTES = {
     "fused-lds": "fused passes with the tile in LDS",
     "per-gate": "one sweep kernel per gate (bit-identical to the oracle)",
 }
+
+
+def specialize_info(lib) -> dict:
+    import ctypes as C
+    p, c, t = C.c_uint32(), C.c_uint32(), C.c_double()
+    lib.qoa_specialize_info(C.byref(p), C.byref(c), C.byref(t))
+    return {"programs": p.value, "compiles": c.value, "compile_seconds": t.value}
 
 
 def log(msg: str) -> None:
@@ -232,6 +244,8 @@ def run_circuit(sv, gates, method: str, steps: int, warmup: int, tag: str):
             sv.run_fused(gates, flags=QOA_FUSE_FMA)
         elif method == "fused-merge":
             sv.run_fused(gates, flags=QOA_FUSE_MERGE_DIAG)
+        elif method == "fused-spec":
+            sv.run_fused(gates, flags=QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE)
         else:
             sv.run_fused(gates)
 
@@ -615,8 +629,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--method", default="fused-merge",
-                    choices=["fused", "fused-fma", "fused-merge", "fused-lds", "per-gate"])
+    ap.add_argument("--method", default="fused-spec",
+                    choices=["fused", "fused-fma", "fused-merge", "fused-spec", "fused-lds",
+                             "per-gate"])
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=20241203)
@@ -655,7 +670,9 @@ def main() -> int:
     log(f"workload: {n} qubits, depth {args.depth}, {len(gates)} gates")
 
     sv = StateVector(n)
+    spec0 = specialize_info(lib)
     elapsed, kinds = run_circuit(sv, gates, args.method, args.steps, args.warmup, "c2")
+    spec1 = specialize_info(lib)
     dom = max(kinds.items(), key=lambda kv: kv[1]["ms"])
     tot_ms = sum(v["ms"] for v in kinds.values())
     tot_b = sum(v["bytes"] for v in kinds.values())
@@ -672,6 +689,13 @@ def main() -> int:
         return want is None or label in want
 
     extras = {}
+    if spec1["compiles"] > spec0["compiles"] or args.method == "fused-spec":
+        extras["c2_specialize"] = {
+            "programs_compiled": spec1["compiles"] - spec0["compiles"],
+            "compile_s": spec1["compile_seconds"] - spec0["compile_seconds"],
+            "note": "hipRTC compile of the circuit's per-pass kernels, in the first (warm-up) "
+                    "call and outside the timed steps; cached per process and on disk by "
+                    "the generated source"}
     accuracy = None
     copy = None
     if wanted("c2"):
@@ -681,7 +705,7 @@ def main() -> int:
         el, kk = run_circuit(ref, gates, "per-gate", 1, 0, "c2 per-gate")
         extras["c2_per-gate"] = {"ms_per_step": el * 1e3, "amp_updates_per_s": amp_updates / el,
                                  "kernels": kk}
-        for other in ("fused", "fused-fma", "fused-merge", "fused-lds"):
+        for other in ("fused", "fused-fma", "fused-merge", "fused-spec", "fused-lds"):
             el, kk = run_circuit(sv, gates, other, 1, 0, f"c2 {other}")
             d, f = sv.compare(ref)
             entry = {"ms_per_step": el * 1e3, "amp_updates_per_s": amp_updates / el,
@@ -796,7 +820,8 @@ def roofline(dom, args, copy=None) -> dict:
         out["frac_of_copy"] = k["gbps"] / copy["gbps"]
     default = (args.qubits, args.depth, args.seed) == (28, 40, 20241203)
     key = {"fused": "fused_pass/28q-d40-ladders",
-           "fused-merge": "fused_pass_merge/28q-d40-ladders"}.get(args.method)
+           "fused-merge": "fused_pass_merge/28q-d40-ladders",
+           "fused-spec": "fused_pass_spec/28q-d40-ladders"}.get(args.method)
     prof = measured_profile(key) if (default and kind == "fused_pass" and key) else None
     if prof:
         out["traffic"] = prof["bytes_per_launch"]

@@ -26,8 +26,12 @@
 #ifndef QOA_HIP_H
 #define QOA_HIP_H
 
+#if defined(__HIPCC_RTC__)  /* kernels compiled at run time see the types only */
+#include <cstdint>
+#else
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -102,6 +106,15 @@ typedef struct qoa_gate {
                                 (tested <= 1e-12) instead of bit-identical.
                                 Implies QOA_FUSE_FMA. Single-state fused
                                 runs only (batches and serving ignore it).  */
+#define QOA_FUSE_SPECIALIZE 64u /* register-resident passes run as kernels
+                                compiled for this plan (hipRTC, on first use;
+                                the code is cached per process by its
+                                source): every op's words are constants, so
+                                the per-op decoding of the op-loop kernel
+                                folds away. Results are bit-identical to the
+                                op-loop kernel with the same other flags.
+                                Compiling takes seconds per circuit
+                                structure; see qoa_specialize_info.        */
 #define QOA_CZ_REF_MASKS 1u  /* CZ/CPHASE use the integer c|t as the bit mask,
                                 as QuantumState::apply_cz does
                                 (src/runtime/quantum_state.rs:338, 356-361).
@@ -277,6 +290,17 @@ int qoa_plan_program(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
 int qoa_plan_pool(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
                   const double* matrices, const qoa_fuse_opts* opts, double* pool_out,
                   uint32_t max_doubles, uint32_t* n_doubles);
+
+/* Specialised passes (QOA_FUSE_SPECIALIZE; no reference counterpart: the
+   reference has no fused passes). qoa_specialize_compile plans the circuit
+   and compiles its per-pass kernels for the architecture `arch` (e.g.
+   "gfx950") into the process-wide cache without loading them: host only,
+   no device needed (a build check, or a warm cache before the first run).
+   qoa_specialize_info reports the cache: programs held, programs compiled
+   so far and the wall time spent compiling. */
+int qoa_specialize_compile(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
+                           const double* matrices, const qoa_fuse_opts* opts, const char* arch);
+int qoa_specialize_info(uint32_t* programs, uint32_t* compiles, double* compile_seconds);
 
 /* Dense 5-qubit blocks on complex64 states (the block-fusion primitive the
    boundary lists as apply_fused_block; no reference counterpart: the

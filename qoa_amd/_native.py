@@ -30,6 +30,7 @@ QOA_CZ_REF_MASKS = 1
 QOA_FUSE_LDS_ONLY = 2
 QOA_FUSE_FMA = 4
 QOA_FUSE_MERGE_DIAG = 16
+QOA_FUSE_SPECIALIZE = 64
 QOA_BATCH_AUTOTUNE = 32
 QOA_BLOCK_NO_RENORM = 8
 
@@ -127,6 +128,8 @@ _SIGS = {
                         I32),
     "qoa_state_compare": ([P, P, DP, DP], I32),
     "qoa_plan_pool": ([U32, P, U32, P, P, P, U32, C.POINTER(U32)], I32),
+    "qoa_specialize_compile": ([U32, P, U32, P, P, C.c_char_p], I32),
+    "qoa_specialize_info": ([C.POINTER(U32), C.POINTER(U32), DP], I32),
     "qoa_plan_program": ([U32, P, U32, P, P, P, U32, P, P, U32, C.POINTER(U32), C.POINTER(U32),
                           C.POINTER(U32)], I32),
     "qoa_timing_enable": ([P, I32], I32),

@@ -53,6 +53,41 @@ COMMON = [
 NEAR_BRANCHES = ["-mllvm", "-amdgpu-s-branch-bits=17"]
 
 
+# Headers that kernels compiled at run time (hipRTC, csrc/specialize.cpp)
+# include, by the name they are included under.
+RTC_HEADERS = [
+    ("jit_device.hpp", "jit_device.hpp"),
+    ("tile_device.hpp", "tile_device.hpp"),
+    ("planner.hpp", "planner.hpp"),
+    ("traj_noise.hpp", "traj_noise.hpp"),
+    ("../../include/qoa_hip.h", "../../include/qoa_hip.h"),
+]
+
+
+def rtc_headers_source() -> Path:
+    """Writes _build/rtc_headers.cpp: the texts of RTC_HEADERS as string
+    literals (kRtcHeaderNames / kRtcHeaderTexts), so the library carries the
+    device code it compiles at run time."""
+    out = BUILD / "rtc_headers.cpp"
+    parts = ["// generated by qoa_amd/build.py from the headers named below", "namespace qoa {",
+             f"extern const int kRtcHeaderCount = {len(RTC_HEADERS)};",
+             "extern const char* const kRtcHeaderNames[] = {"]
+    parts += [f'    "{name}",' for name, _ in RTC_HEADERS]
+    parts += ["};", "extern const char* const kRtcHeaderTexts[] = {"]
+    for _, rel in RTC_HEADERS:
+        text = (CSRC / rel).read_text()
+        delim = "QOAHDR"
+        assert f"){delim}\"" not in text
+        # long texts as concatenated literals (compilers limit one literal)
+        chunks = [text[i:i + 12000] for i in range(0, len(text), 12000)] or [""]
+        parts.append("    " + " ".join(f'R"{delim}({c}){delim}"' for c in chunks) + ",")
+    parts += ["};", "}  // namespace qoa", ""]
+    text = "\n".join(parts)
+    if not out.exists() or out.read_text() != text:
+        out.write_text(text)
+    return out
+
+
 def sources() -> list[Path]:
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
 
@@ -71,11 +106,12 @@ def _compile(src: Path) -> Path:
 
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
-    srcs = sources()
+    srcs = sources() + [rtc_headers_source()]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp),
+           "-lhiprtc"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

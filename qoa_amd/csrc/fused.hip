@@ -12,6 +12,7 @@
 
 #include "planner.hpp"
 #include "qoa_internal.hpp"
+#include "specialize.hpp"
 #include "tile_device.hpp"
 
 #pragma clang fp contract(off)
@@ -179,6 +180,10 @@ struct FusedCache {
   FusedPlan plan;
   TileOp* d_ops = nullptr;
   double* d_pool = nullptr;
+  // QOA_FUSE_SPECIALIZE: one compiled kernel per pass, for the arithmetic
+  // (spec_mode bit 0: FMA) and index width (bit 1: 64-bit) they were made for
+  SpecKernels spec;
+  int spec_mode = -1;
 };
 
 void fused_cache_free(qoa_state* s) {
@@ -230,6 +235,20 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
   }
   const FusedPlan& P = c->plan;
   const uint32_t k = P.k;
+  // merged diagonal runs exist in the FMA instantiation only
+  const bool fma = (opt.flags & (QOA_FUSE_FMA | QOA_FUSE_MERGE_DIAG)) != 0;
+  bool spec = false;
+  if (opt.flags & QOA_FUSE_SPECIALIZE) {
+    spec = true;
+    for (const PassPlan& pp : P.passes) spec = spec && pp.reg_mode;
+    const int mode = (fma ? 1 : 0) | (use32(s) ? 0 : 2);
+    if (spec && c->spec_mode != mode) {
+      c->spec_mode = -1;
+      int rc = specialize_plan(P, fma, !use32(s), &c->spec);
+      if (rc) return rc;
+      c->spec_mode = mode;
+    }
+  }
   const size_t lds_bytes = sizeof(double2) << k;
   static std::once_flag once;
   std::call_once(once, [] {
@@ -260,10 +279,14 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
       // changes (a quarter of a 12-qubit tile)
       const unsigned threads = 1u << (k - kRegBits);
       const size_t lbytes = sizeof(double2) * reg_xchg_elems(k, false);
-      // merged diagonal runs exist in the FMA instantiation only
-      const bool fma = (opt.flags & (QOA_FUSE_FMA | QOA_FUSE_MERGE_DIAG)) != 0;
       const dim3 grid((unsigned)tiles), block(threads);
-      if (use32(s)) {
+      if (spec) {
+        const double* pool = c->d_pool;
+        void* args[] = {&a, &ops, &pool};
+        QOA_HIP_CHECK(hipModuleLaunchKernel(c->spec.fn[&pp - P.passes.data()], (unsigned)tiles, 1,
+                                            1, threads, 1, 1, (unsigned)lbytes, s->stream, args,
+                                            nullptr));
+      } else if (use32(s)) {
         if (fma)
           hipLaunchKernelGGL((k_fused_reg<uint32_t, true>), grid, block, lbytes, s->stream, a, ops,
                              nops, c->d_pool, geo);

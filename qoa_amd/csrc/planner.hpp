@@ -1,7 +1,9 @@
 // planner.hpp — gate list -> LDS-tile passes (host side of the fused path).
 #pragma once
 #include <cstdint>
+#if !defined(__HIPCC_RTC__)  // kernels compiled at run time see the op records only
 #include <vector>
+#endif
 
 #include "../../include/qoa_hip.h"
 
@@ -114,6 +116,7 @@ struct alignas(16) TileOp {
 };
 static_assert(sizeof(TileOp) == 112, "TileOp layout");
 
+#if !defined(__HIPCC_RTC__)
 struct PassPlan {
   uint32_t k;                 // tile qubits
   uint32_t reg_mode;          // 1: register-resident kernel, 0: LDS-resident kernel
@@ -132,5 +135,6 @@ struct FusedPlan {
 struct PlanOptions;
 int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
                const PlanOptions& opt, FusedPlan* plan);
+#endif
 
 }  // namespace qoa

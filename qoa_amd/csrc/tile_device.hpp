@@ -7,7 +7,9 @@
 // multiply-adds; it is selected per call and is not bit-identical.
 #pragma once
 #include "planner.hpp"
+#if !defined(__HIPCC_RTC__)
 #include "qoa_internal.hpp"
+#endif
 #include "traj_noise.hpp"
 
 #include <type_traits>
@@ -1292,13 +1294,13 @@ __device__ __forceinline__ void reg_pauli(double2 (&v)[kR], uint32_t code, uint3
 // w11 = psel | qphys[0] << 8 | qphys[1] << 16 | qphys[2] << 24,
 // w12..w27 = c[0..7] (the offsets below fix this layout).
 constexpr int kOpWords = 28;
-static_assert(sizeof(TileOp) == 4 * kOpWords && offsetof(TileOp, ins) == 4 &&
-                  offsetof(TileOp, rsel) == 8 && offsetof(TileOp, rt) == 9 &&
-                  offsetof(TileOp, nq) == 11 && offsetof(TileOp, tl) == 12 &&
-                  offsetof(TileOp, gmask) == 16 && offsetof(TileOp, gval) == 24 &&
-                  offsetof(TileOp, aux) == 32 && offsetof(TileOp, gi) == 40 &&
-                  offsetof(TileOp, psel) == 44 && offsetof(TileOp, qphys) == 45 &&
-                  offsetof(TileOp, c) == 48,
+static_assert(sizeof(TileOp) == 4 * kOpWords && __builtin_offsetof(TileOp, ins) == 4 &&
+                  __builtin_offsetof(TileOp, rsel) == 8 && __builtin_offsetof(TileOp, rt) == 9 &&
+                  __builtin_offsetof(TileOp, nq) == 11 && __builtin_offsetof(TileOp, tl) == 12 &&
+                  __builtin_offsetof(TileOp, gmask) == 16 && __builtin_offsetof(TileOp, gval) == 24 &&
+                  __builtin_offsetof(TileOp, aux) == 32 && __builtin_offsetof(TileOp, gi) == 40 &&
+                  __builtin_offsetof(TileOp, psel) == 44 && __builtin_offsetof(TileOp, qphys) == 45 &&
+                  __builtin_offsetof(TileOp, c) == 48,
               "TileOp word layout");
 
 struct OpRegs {

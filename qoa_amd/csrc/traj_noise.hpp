@@ -1,7 +1,9 @@
 // traj_noise.hpp — the trajectory-noise draw shared by the pass kernels, the
 // draw-table kernel and the host's plain per-gate trajectory loop.
 #pragma once
+#if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>
+#endif
 
 #include <cstdint>
 
