@@ -59,13 +59,18 @@ __device__ __forceinline__ void spec_load(SpecTile<I>& S, double2* __restrict__ 
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
 }
 
-// Stores the tile from its final layout.
+// Stores the tile from its final layout. The thread's local bits pass
+// through an empty asm statement: in a pass without layout changes the store
+// addresses equal the load addresses, and the compiler would otherwise keep
+// those 16 addresses live (32 VGPRs) through the whole pass.
 template <typename I>
 __device__ __forceinline__ void spec_store(SpecTile<I>& S, double2* __restrict__ a,
                                            const Geo& geo) {
   const int k = S.k;
+  uint32_t thr = S.L.thr_l;
+  asm volatile("" : "+v"(thr));
   I gthr = 0;
-  for (int j = 0; j < k; ++j) gthr |= (I)((S.L.thr_l >> j) & 1u) << spos_at(geo, j);
+  for (int j = 0; j < k; ++j) gthr |= (I)((thr >> j) & 1u) << spos_at(geo, j);
   I gb[kRegBits];
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j) gb[j] = (I)1 << spos_at(geo, S.rpos[j]);
@@ -126,18 +131,20 @@ __device__ __forceinline__ void spec_layout(SpecTile<I>& S, const TileOp* __rest
 // A merged diagonal run term by term: the factor at c applies where the tile
 // condition (base & gm) == gv holds and the thread's local bits hold tl; a
 // group's product then multiplies the registers whose slots rs are all 1.
+// A term with a tile condition multiplies by (1, 0) where it does not hold
+// (exactly the value the op loop's skipped product leaves; no branch, so the
+// straight-line kernel keeps its register assignment across the term).
 template <typename I>
 __device__ __forceinline__ void spec_term(SpecTile<I>& S, double2& acc,
                                           const double* __restrict__ c, const uint32_t tl,
                                           const uint64_t gm, const uint64_t gv) {
-  if (((uint64_t)S.base & gm) == gv) {
-    const double cr = c[0], ci = c[1];
-    if (tl == 0) {
-      ip_mulc<true, true>(acc, make_double2(cr, ci));
-    } else {
-      const bool on = (S.L.thr_l & tl) == tl;
-      ip_mulc<true>(acc, make_double2(on ? cr : 1.0, on ? ci : 0.0));
-    }
+  const bool t = gm == 0 || ((uint64_t)S.base & gm) == gv;  // uniform
+  const double cr = t ? c[0] : 1.0, ci = t ? c[1] : 0.0;
+  if (tl == 0) {
+    ip_mulc<true, true>(acc, make_double2(cr, ci));
+  } else {
+    const bool on = (S.L.thr_l & tl) == tl;
+    ip_mulc<true>(acc, make_double2(on ? cr : 1.0, on ? ci : 0.0));
   }
 }
 

@@ -218,6 +218,27 @@ std::string rtc_log(hiprtcProgram prog) {
   return log;
 }
 
+// The library's device flags: unfused FP64 products, no packed FP32, and the
+// two code generation options the register kernels rely on.
+const char* const kRtcOptions[] = {"-O3", "-std=c++17", "-ffp-contract=off",
+                                   "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                                   "-mllvm", "-simplifycfg-sink-common=false",
+                                   "-mllvm", "-structurizecfg-skip-uniform-regions=true"};
+constexpr int kRtcOptionCount = (int)(sizeof(kRtcOptions) / sizeof(kRtcOptions[0]));
+
+// Hash of what a compiled program depends on besides its own source: the
+// embedded device headers and the options (so a library built from changed
+// device code never takes an older code object from the disk cache).
+uint64_t toolchain_key() {
+  static const uint64_t key = [] {
+    uint64_t h = 0xCBF29CE484222325ULL;
+    for (int i = 0; i < kRtcHeaderCount; ++i) h = fnv64(kRtcHeaderNames[i], fnv64(kRtcHeaderTexts[i], h));
+    for (int i = 0; i < kRtcOptionCount; ++i) h = fnv64(kRtcOptions[i], h);
+    return h;
+  }();
+  return key;
+}
+
 // Compiles one source; returns an error message or "".
 std::string rtc_compile(const std::string& src, const std::string& arch, std::vector<char>* code) {
   hiprtcProgram prog;
@@ -225,13 +246,9 @@ std::string rtc_compile(const std::string& src, const std::string& arch, std::ve
                           kRtcHeaderNames) != HIPRTC_SUCCESS)
     return "hiprtcCreateProgram failed";
   const std::string off = "--offload-arch=" + arch;
-  // the library's device flags: unfused FP64 products, no packed FP32, and
-  // the two code generation options the register kernels rely on
-  const char* opts[] = {off.c_str(), "-O3", "-std=c++17", "-ffp-contract=off",
-                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-                        "-mllvm", "-simplifycfg-sink-common=false",
-                        "-mllvm", "-structurizecfg-skip-uniform-regions=true"};
-  const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  std::vector<const char*> opts{off.c_str()};
+  opts.insert(opts.end(), kRtcOptions, kRtcOptions + kRtcOptionCount);
+  const hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
   std::string err;
   if (r != HIPRTC_SUCCESS) {
     err = std::string("hipRTC: ") + hiprtcGetErrorString(r) + "\n" + rtc_log(prog).substr(0, 4000);
@@ -320,7 +337,7 @@ int compile_plan(const FusedPlan& P, bool fma, bool idx64, const std::string& ar
     if (!pp.reg_mode)
       return set_error(QOA_ERR_UNSUPPORTED, "specialised passes need register-mode plans");
   const std::string src = specialized_source(P, 0, (uint32_t)P.passes.size(), fma, idx64);
-  const uint64_t key = fnv64(arch, fnv64(src));
+  const uint64_t key = fnv64(arch, fnv64(src, toolchain_key()));
   {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_modules.find(key);

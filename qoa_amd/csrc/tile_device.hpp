@@ -552,6 +552,32 @@ __device__ __forceinline__ void ip_swap2(double2& x, double2& y) {
   ip_swap(x.y, y.y);
 }
 
+// x <-> y in the lanes of `mask` only: the swaps run with exec narrowed to
+// those lanes (and restored in the same statement), so no register beyond x
+// and y is written (selects would need a copy of every word).
+__device__ __forceinline__ void ip_cswap2(double2& x, double2& y, uint64_t mask) {
+  uint64_t ux = __builtin_bit_cast(uint64_t, x.x), uy = __builtin_bit_cast(uint64_t, y.x);
+  uint64_t vx = __builtin_bit_cast(uint64_t, x.y), vy = __builtin_bit_cast(uint64_t, y.y);
+  uint32_t a0 = (uint32_t)ux, a1 = (uint32_t)(ux >> 32), b0 = (uint32_t)uy, b1 = (uint32_t)(uy >> 32);
+  uint32_t c0 = (uint32_t)vx, c1 = (uint32_t)(vx >> 32), d0 = (uint32_t)vy, d1 = (uint32_t)(vy >> 32);
+  uint64_t save;
+  asm volatile(
+      "s_and_saveexec_b64 %[s], %[m]\n\t"
+      "v_swap_b32 %[a0], %[b0]\n\t"
+      "v_swap_b32 %[a1], %[b1]\n\t"
+      "v_swap_b32 %[c0], %[d0]\n\t"
+      "v_swap_b32 %[c1], %[d1]\n\t"
+      "s_mov_b64 exec, %[s]"
+      : [a0] "+v"(a0), [a1] "+v"(a1), [b0] "+v"(b0), [b1] "+v"(b1), [c0] "+v"(c0),
+        [c1] "+v"(c1), [d0] "+v"(d0), [d1] "+v"(d1), [s] "=&s"(save)
+      : [m] "s"(mask)
+      : "scc");
+  x = make_double2(__builtin_bit_cast(double, ((uint64_t)a1 << 32) | a0),
+                   __builtin_bit_cast(double, ((uint64_t)c1 << 32) | c0));
+  y = make_double2(__builtin_bit_cast(double, ((uint64_t)b1 << 32) | b0),
+                   __builtin_bit_cast(double, ((uint64_t)d1 << 32) | d0));
+}
+
 // x = lanes of `mask` ? n : x
 __device__ __forceinline__ void ip_select(double& x, double n, uint64_t mask) {
   uint64_t ux = __builtin_bit_cast(uint64_t, x), un = __builtin_bit_cast(uint64_t, n);
@@ -711,10 +737,8 @@ __device__ __forceinline__ void reg_pair_sel(double2 (&v)[kR], const Coef4& k, u
     if (r & (1 << J)) continue;
     if (((uint32_t)r & rsel) != rsel) continue;  // uniform
     const int r1 = r | (1 << J);
-    if constexpr (KIND == TK_X) {  // a conditional swap: two selects per word
-      const double2 x = v[r], y = v[r1];
-      v[r] = tok ? y : x;
-      v[r1] = tok ? x : y;
+    if constexpr (KIND == TK_X) {  // a conditional swap in the lanes of tok
+      ip_cswap2(v[r], v[r1], mask);
     } else {
       double2 o0, o1;
       pair_math<KIND, F>(v[r], v[r1], k, o0, o1);
@@ -824,9 +848,7 @@ __device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel, uint32
     if (tl == 0) {  // uniform
       ip_swap2(v[r | (1 << A)], v[r | (1 << B)]);
     } else {
-      const double2 u = v[r | (1 << B)], w = v[r | (1 << A)];
-      ip_select2(v[r | (1 << B)], w, mask);
-      ip_select2(v[r | (1 << A)], u, mask);
+      ip_cswap2(v[r | (1 << A)], v[r | (1 << B)], mask);
     }
   }
 }
