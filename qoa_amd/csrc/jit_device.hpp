@@ -59,13 +59,14 @@ __device__ __forceinline__ void spec_load(SpecTile<I>& S, double2* __restrict__ 
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
 }
 
-// Stores the tile from its final layout. The thread's local bits pass
-// through an empty asm statement: in a pass without layout changes the store
-// addresses equal the load addresses, and the compiler would otherwise keep
-// those 16 addresses live (32 VGPRs) through the whole pass.
+// Stores the tile from its final layout (logical register r in physical
+// register pm(pi, r)). The thread's local bits pass through an empty asm
+// statement: in a pass without layout changes the store addresses equal the
+// load addresses, and the compiler would otherwise keep those 16 addresses
+// live (32 VGPRs) through the whole pass.
 template <typename I>
 __device__ __forceinline__ void spec_store(SpecTile<I>& S, double2* __restrict__ a,
-                                           const Geo& geo) {
+                                           const Geo& geo, uint64_t pi) {
   const int k = S.k;
   uint32_t thr = S.L.thr_l;
   asm volatile("" : "+v"(thr));
@@ -80,7 +81,7 @@ __device__ __forceinline__ void spec_store(SpecTile<I>& S, double2* __restrict__
 #pragma unroll
     for (int j = 0; j < kRegBits; ++j)
       if ((r >> j) & 1) o |= gb[j];
-    st_stream(a + o, S.v[r]);
+    st_stream(a + o, S.v[pm(pi, r)]);
   }
 }
 
@@ -115,16 +116,17 @@ __device__ __forceinline__ void spec_rz_thread(SpecTile<I>& S, double2 m00, doub
 }
 
 // Whole-tile exchange op p to the register bits in w1 (a byte per slot),
-// rounds keyed by slots A (and B), Q = reg_xchg_split(k, false).
+// rounds keyed by slots A (and B), Q = reg_xchg_split(k, false); logical
+// register r is physical register pm(pi, r) on both sides.
 template <int Q, int A, int B, typename I>
 __device__ __forceinline__ void spec_layout(SpecTile<I>& S, const TileOp* __restrict__ p,
-                                            uint32_t w1) {
+                                            uint32_t w1, uint64_t pi) {
   const uint32_t* w = reinterpret_cast<const uint32_t*>(p) + kCoefWord;
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j) S.rpos[j] = (uint8_t)(w1 >> (8 * j));
   RegLayout N;
   set_layout_sorted(N, S.tid, S.rpos, w[kXchgSorted]);
-  xchg_op<Q, A, B>(S.v, S.L, N, w, S.X, true);
+  xchg_op<Q, A, B>(S.v, S.L, N, w, S.X, true, pi);
   S.L = N;
 }
 
@@ -149,8 +151,8 @@ __device__ __forceinline__ void spec_term(SpecTile<I>& S, double2& acc,
 }
 
 template <int RS, typename I>
-__device__ __forceinline__ void spec_group(SpecTile<I>& S, double2& acc) {
-  reg_phase<RS, true, false>(S.v, acc);
+__device__ __forceinline__ void spec_group(SpecTile<I>& S, double2& acc, uint64_t pi) {
+  reg_phase<RS, true, false>(S.v, acc, pi);
   acc = make_double2(1.0, 0.0);
 }
 

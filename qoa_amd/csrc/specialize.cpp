@@ -61,12 +61,50 @@ void append_u64(std::string* out, uint64_t x) {
   *out += b;
 }
 
+// Logical register r of the tile lives in physical register perm[r]: X,
+// CNOT and SWAP ops on register slots (without thread or tile conditions)
+// only permute this map, and every other op is emitted with it.
+struct Perm {
+  uint8_t p[kRegAmps];
+  Perm() {
+    for (int r = 0; r < kRegAmps; ++r) p[r] = (uint8_t)r;
+  }
+  uint64_t word() const {
+    uint64_t w = 0;
+    for (int r = 0; r < kRegAmps; ++r) w |= (uint64_t)p[r] << (4 * r);
+    return w;
+  }
+};
+
+// Applies op to the permutation instead of emitting code, when it is a
+// register permutation without thread or tile conditions; returns whether it
+// did. (reg_pair<slot, rsel, TK_X> and reg_swap<A, B> move exactly these
+// registers.)
+bool permute(const TileOp& op, Perm* pm) {
+  if (op.tl != 0 || op.gmask != 0 || op.gval != 0) return false;
+  if (op.kind == TK_X) {
+    const uint32_t j = std::min<uint32_t>(op.ins[0], 3u), rs = op.rsel & ~(1u << j);
+    for (uint32_t r = 0; r < (uint32_t)kRegAmps; ++r)
+      if (!(r & (1u << j)) && (r & rs) == rs) std::swap(pm->p[r], pm->p[r | (1u << j)]);
+    return true;
+  }
+  if (op.kind == TK_SWAP) {
+    const uint32_t a = std::min(op.ins[0], op.ins[1]), c = std::max(op.ins[0], op.ins[1]);
+    if (a == c || c > 3) return false;
+    const uint32_t ma = 1u << a, mc = 1u << c, rs = op.rsel;
+    for (uint32_t r = 0; r < (uint32_t)kRegAmps; ++r)
+      if (!(r & (ma | mc)) && (r & rs) == rs) std::swap(pm->p[r | ma], pm->p[r | mc]);
+    return true;
+  }
+  return false;
+}
+
 // The call reg_op (or reg_tile's exchange) makes for op `op`, record index
 // i of its pass, with everything that depends on the op's words resolved
 // here as reg_op resolves it on the device (kind, 2x2 shape, register slot,
-// register and thread-bit controls).
-std::string op_call(const TileOp& op, uint32_t i, uint32_t k, bool fma) {
-  char b[256];
+// register and thread-bit controls), on registers permuted by pi.
+std::string op_call(const TileOp& op, uint32_t i, uint32_t k, bool fma, uint64_t pi) {
+  char b[320];
   const char* F = fma ? "true" : "false";
   const uint32_t rsel = op.rsel, tl = op.tl;
   const uint32_t j = std::min<uint32_t>(op.ins[0], 3u);  // a 2x2 op's register slot
@@ -83,37 +121,39 @@ std::string op_call(const TileOp& op, uint32_t i, uint32_t k, bool fma) {
         kind = op.kind == TK_X ? "TK_X" : "TK_Y";
       }
       if (tl == 0 && rsel == 0)
-        std::snprintf(b, sizeof(b), "reg_pair<%u, 0, %s, %s>(S.v, spec_coef4(ops + %u));", j, kind,
-                      F, i);
+        std::snprintf(b, sizeof(b), "reg_pair<%u, 0, %s, %s>(S.v, spec_coef4(ops + %u), %lluull);", j,
+                      kind, F, i, (unsigned long long)pi);
       else if (op.kind == TK_X && tl == 0)
-        std::snprintf(b, sizeof(b), "reg_pair<%u, %u, TK_X, %s>(S.v, spec_coef4(ops + %u));", j,
-                      rsel, F, i);
+        std::snprintf(b, sizeof(b), "reg_pair<%u, %u, TK_X, %s>(S.v, spec_coef4(ops + %u), %lluull);",
+                      j, rsel, F, i, (unsigned long long)pi);
       else
         std::snprintf(b, sizeof(b),
-                      "reg_pair_sel<%u, %s, %s>(S.v, spec_coef4(ops + %u), %uu, spec_tok(S, %uu));",
-                      j, shaped ? "TK_M" : kind, F, i, rsel, tl);
+                      "reg_pair_sel<%u, %s, %s>(S.v, spec_coef4(ops + %u), %uu, spec_tok(S, %uu),"
+                      " %lluull);",
+                      j, shaped ? "TK_M" : kind, F, i, rsel, tl, (unsigned long long)pi);
     } break;
     case TK_DNEG:
       if (tl == 0)
-        std::snprintf(b, sizeof(b), "reg_neg<%u>(S.v);", rsel);
+        std::snprintf(b, sizeof(b), "reg_neg<%u>(S.v, %lluull);", rsel, (unsigned long long)pi);
       else
-        std::snprintf(b, sizeof(b), "reg_scale<%u>(S.v, spec_tok(S, %uu) ? -1.0 : 1.0);", rsel, tl);
+        std::snprintf(b, sizeof(b), "reg_scale<%u>(S.v, spec_tok(S, %uu) ? -1.0 : 1.0, %lluull);",
+                      rsel, tl, (unsigned long long)pi);
       break;
     case TK_DPHASE:
       if (tl == 0)
-        std::snprintf(b, sizeof(b), "reg_phase<%u, %s, true>(S.v, spec_c2(ops + %u, 0));", rsel, F,
-                      i);
+        std::snprintf(b, sizeof(b), "reg_phase<%u, %s, true>(S.v, spec_c2(ops + %u, 0), %lluull);",
+                      rsel, F, i, (unsigned long long)pi);
       else
         std::snprintf(b, sizeof(b),
                       "reg_phase<%u, %s, false>(S.v, sel2(spec_tok(S, %uu), spec_c2(ops + %u, 0),"
-                      " make_double2(1.0, 0.0)));",
-                      rsel, F, tl, i);
+                      " make_double2(1.0, 0.0)), %lluull);",
+                      rsel, F, tl, i, (unsigned long long)pi);
       break;
     case TK_RZ:
       if (op.rt < kRegBits)
         std::snprintf(b, sizeof(b),
-                      "reg_rz<%u, %s>(S.v, spec_c2(ops + %u, 0), spec_c2(ops + %u, 1));", op.rt, F,
-                      i, i);
+                      "reg_rz<%u, %s>(S.v, spec_c2(ops + %u, 0), spec_c2(ops + %u, 1), %lluull);",
+                      op.rt, F, i, i, (unsigned long long)pi);
       else
         std::snprintf(b, sizeof(b),
                       "spec_rz_thread<%s>(S, spec_c2(ops + %u, 0), spec_c2(ops + %u, 1), %uu);", F,
@@ -124,11 +164,12 @@ std::string op_call(const TileOp& op, uint32_t i, uint32_t k, bool fma) {
       const uint32_t a0 = std::min(s0, s1), b0 = std::max(s0, s1);
       // reg_op's switch over the slot pair (its default case is slots 2, 3)
       const bool known = (a0 == 0 && b0 >= 1 && b0 <= 3) || (a0 == 1 && (b0 == 2 || b0 == 3));
-      std::snprintf(b, sizeof(b), "reg_swap<%u, %u>(S.v, %uu, %uu, spec_tok(S, %uu));",
-                    known ? a0 : 2u, known ? b0 : 3u, rsel, tl, tl);
+      std::snprintf(b, sizeof(b), "reg_swap<%u, %u>(S.v, %uu, %uu, spec_tok(S, %uu), %lluull);",
+                    known ? a0 : 2u, known ? b0 : 3u, rsel, tl, tl, (unsigned long long)pi);
     } break;
     case TK_MAT4:
-      std::snprintf(b, sizeof(b), "reg_mat4<2, 3, %s>(S.v, pool + %u);", F, op.aux);
+      std::snprintf(b, sizeof(b), "reg_mat4<2, 3, %s>(S.v, pool + %u, %lluull);", F, op.aux,
+                    (unsigned long long)pi);
       break;
     case TK_LAYOUT: {  // relayout's choice of the round-keying slots
       const uint32_t q = reg_xchg_split(k, false), a = op.t, c = op.t2;
@@ -144,7 +185,8 @@ std::string op_call(const TileOp& op, uint32_t i, uint32_t k, bool fma) {
       }
       uint32_t w1;
       std::memcpy(&w1, op.ins, sizeof(w1));
-      std::snprintf(b, sizeof(b), "spec_layout<%u, %u, %u>(S, ops + %u, %uu);", q, A, B, i, w1);
+      std::snprintf(b, sizeof(b), "spec_layout<%u, %u, %u>(S, ops + %u, %uu, %lluull);", q, A, B,
+                    i, w1, (unsigned long long)pi);
     } break;
     default:
       b[0] = '\0';
@@ -173,8 +215,11 @@ void pass_source(const FusedPlan& P, uint32_t p, bool fma, bool idx64, std::stri
                 "  spec_load(S, a, geo, lds, (%s)blockIdx.x);\n",
                 threads, p, k, spw[0], spw[1], spw[2], spw[3], I, I);
   *out += head;
+  Perm perm;
   for (uint32_t i = pp.op_begin; i < pp.op_end; ++i) {
     const TileOp& op = P.ops[i];
+    if (permute(op, &perm)) continue;
+    const uint64_t pi = perm.word();
     std::string call;
     if (op.kind == TK_DIAGN) {  // terms at pool[aux], lmask of them (FMA kernels only)
       if (!fma) continue;
@@ -191,11 +236,15 @@ void pass_source(const FusedPlan& P, uint32_t p, bool fma, bool idx64, std::stri
         call += ", ";
         append_u64(&call, d.gval);
         call += ");\n";
-        if (d.rs & kDiagGroupEnd) call += "    spec_group<" + std::to_string(d.rs & 15u) + ">(S, acc);\n";
+        if (d.rs & kDiagGroupEnd) {
+          call += "    spec_group<" + std::to_string(d.rs & 15u) + ">(S, acc, ";
+          append_u64(&call, pi);
+          call += ");\n";
+        }
       }
       call += "  }";
     } else {
-      call = op_call(op, i - pp.op_begin, k, fma);
+      call = op_call(op, i - pp.op_begin, k, fma, pi);
     }
     if (op.kind != TK_LAYOUT && (op.gmask || op.gval)) {  // a tile condition (outside qubits)
       *out += "  if (((uint64_t)S.base & ";
@@ -207,7 +256,9 @@ void pass_source(const FusedPlan& P, uint32_t p, bool fma, bool idx64, std::stri
       *out += "  " + call + "\n";
     }
   }
-  *out += "  spec_store(S, a, geo);\n}\n";
+  *out += "  spec_store(S, a, geo, ";
+  append_u64(out, perm.word());
+  *out += ");\n}\n";
 }
 
 std::string rtc_log(hiprtcProgram prog) {

@@ -123,6 +123,14 @@ __device__ __forceinline__ double2 macc(double2 acc, double2 a, double2 x) {
 // thread bits (tl, one per-thread test per op).
 constexpr int kR = kRegAmps;
 
+// A register permutation: logical register r (its slot bits) lives in
+// physical register (pi >> 4r) & 15. The op-loop kernels always pass the
+// identity; specialised kernels pass the permutation their X, CNOT and SWAP
+// ops on register slots have accumulated instead of moving data (the
+// primitives below take it as an argument that is a literal at every call).
+constexpr uint64_t kPermId = 0xFEDCBA9876543210ULL;
+__host__ __device__ constexpr int pm(uint64_t pi, int r) { return (int)((pi >> (4 * r)) & 15u); }
+
 struct RegLayout {
   uint32_t rb[kRegBits];  // 1 << rpos[j] (uniform)
   uint32_t thr_l;         // this thread's local bits, register bits clear
@@ -683,16 +691,17 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
 // 2x2 op on register slot J for the registers whose slots RS are all 1,
 // two pairs per asm statement, pairs in ascending register order.
 template <int J, int RS, int KIND, bool F>
-__device__ __forceinline__ void reg_pair(double2 (&v)[kR], const Coef4& k) {
+__device__ __forceinline__ void reg_pair(double2 (&v)[kR], const Coef4& k, uint64_t pi = kPermId) {
   constexpr int M = (1 << J) | RS, V = RS & ~(1 << J), n = n_regs(M, V);
   static_for<0, n / 2>([&](auto i) {
     constexpr int r0 = nth_reg(M, V, 2 * decltype(i)::value);
     constexpr int r1 = nth_reg(M, V, 2 * decltype(i)::value + 1);
-    pair_inplace2<KIND, F>(v[r0], v[r0 | (1 << J)], v[r1], v[r1 | (1 << J)], k);
+    pair_inplace2<KIND, F>(v[pm(pi, r0)], v[pm(pi, r0 | (1 << J))], v[pm(pi, r1)],
+                           v[pm(pi, r1 | (1 << J))], k);
   });
   if constexpr (n & 1) {
     constexpr int r = nth_reg(M, V, n - 1);
-    pair_inplace<KIND, F>(v[r], v[r | (1 << J)], k);
+    pair_inplace<KIND, F>(v[pm(pi, r)], v[pm(pi, r | (1 << J))], k);
   }
 }
 
@@ -730,7 +739,7 @@ __device__ __forceinline__ void reg_pair_rs(double2 (&v)[kR], const Coef4& k, ui
 // not all set keep their values (tok == false).
 template <int J, int KIND, bool F>
 __device__ __forceinline__ void reg_pair_sel(double2 (&v)[kR], const Coef4& k, uint32_t rsel,
-                                             bool tok) {
+                                             bool tok, uint64_t pi = kPermId) {
   const uint64_t mask = __builtin_amdgcn_ballot_w64(tok);
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
@@ -738,12 +747,12 @@ __device__ __forceinline__ void reg_pair_sel(double2 (&v)[kR], const Coef4& k, u
     if (((uint32_t)r & rsel) != rsel) continue;  // uniform
     const int r1 = r | (1 << J);
     if constexpr (KIND == TK_X) {  // a conditional swap in the lanes of tok
-      ip_cswap2(v[r], v[r1], mask);
+      ip_cswap2(v[pm(pi, r)], v[pm(pi, r1)], mask);
     } else {
       double2 o0, o1;
-      pair_math<KIND, F>(v[r], v[r1], k, o0, o1);
-      ip_select2(v[r], o0, mask);
-      ip_select2(v[r1], o1, mask);
+      pair_math<KIND, F>(v[pm(pi, r)], v[pm(pi, r1)], k, o0, o1);
+      ip_select2(v[pm(pi, r)], o0, mask);
+      ip_select2(v[pm(pi, r1)], o1, mask);
     }
   }
 }
@@ -783,24 +792,24 @@ __device__ __forceinline__ void reg_pair_any(double2 (&v)[kR], const Coef4& k, u
 // Factor f on the registers r with (r & M) == V, four (or two, or one) per
 // asm statement, in ascending register order.
 template <int M, int V, bool F, bool S>
-__device__ __forceinline__ void reg_mulc_set(double2 (&v)[kR], double2 f) {
+__device__ __forceinline__ void reg_mulc_set(double2 (&v)[kR], double2 f, uint64_t pi = kPermId) {
   constexpr int n = n_regs(M, V);
   static_for<0, n / 4>([&](auto i) {
     constexpr int b = 4 * decltype(i)::value;
-    ip_mulc4<F, S>(v[nth_reg(M, V, b)], v[nth_reg(M, V, b + 1)], v[nth_reg(M, V, b + 2)],
-                   v[nth_reg(M, V, b + 3)], f);
+    ip_mulc4<F, S>(v[pm(pi, nth_reg(M, V, b))], v[pm(pi, nth_reg(M, V, b + 1))],
+                   v[pm(pi, nth_reg(M, V, b + 2))], v[pm(pi, nth_reg(M, V, b + 3))], f);
   });
   if constexpr (n % 4 >= 2) {
     constexpr int b = n - n % 4;
-    ip_mulc2<F, S>(v[nth_reg(M, V, b)], v[nth_reg(M, V, b + 1)], f);
+    ip_mulc2<F, S>(v[pm(pi, nth_reg(M, V, b))], v[pm(pi, nth_reg(M, V, b + 1))], f);
   }
-  if constexpr (n % 2) ip_mulc1<F, S>(v[nth_reg(M, V, n - 1)], f);
+  if constexpr (n % 2) ip_mulc1<F, S>(v[pm(pi, nth_reg(M, V, n - 1))], f);
 }
 
 // Diagonal factor f on the registers whose slots RS are all 1.
 template <int RS, bool F, bool S>
-__device__ __forceinline__ void reg_phase(double2 (&v)[kR], double2 f) {
-  reg_mulc_set<RS, RS, F, S>(v, f);
+__device__ __forceinline__ void reg_phase(double2 (&v)[kR], double2 f, uint64_t pi = kPermId) {
+  reg_mulc_set<RS, RS, F, S>(v, f, pi);
 }
 
 template <bool F, bool S>
@@ -811,23 +820,24 @@ __device__ __forceinline__ void reg_phase_rs(double2 (&v)[kR], double2 f, uint32
 // RZ with its target in register slot RT: the factor of register r is
 // m11 or m00 by bit RT of r (uniform).
 template <int RT, bool F>
-__device__ __forceinline__ void reg_rz(double2 (&v)[kR], double2 m00, double2 m11) {
-  reg_mulc_set<1 << RT, 0, F, true>(v, m00);
-  reg_mulc_set<1 << RT, 1 << RT, F, true>(v, m11);
+__device__ __forceinline__ void reg_rz(double2 (&v)[kR], double2 m00, double2 m11,
+                                       uint64_t pi = kPermId) {
+  reg_mulc_set<1 << RT, 0, F, true>(v, m00, pi);
+  reg_mulc_set<1 << RT, 1 << RT, F, true>(v, m11, pi);
 }
 
 template <int RS>
-__device__ __forceinline__ void reg_scale(double2 (&v)[kR], double s) {
+__device__ __forceinline__ void reg_scale(double2 (&v)[kR], double s, uint64_t pi = kPermId) {
 #pragma unroll
   for (int r = 0; r < kR; ++r)
-    if ((r & RS) == RS) ip_scale(v[r], s);
+    if ((r & RS) == RS) ip_scale(v[pm(pi, r)], s);
 }
 
 template <int RS>
-__device__ __forceinline__ void reg_neg(double2 (&v)[kR]) {
+__device__ __forceinline__ void reg_neg(double2 (&v)[kR], uint64_t pi = kPermId) {
 #pragma unroll
   for (int r = 0; r < kR; ++r)
-    if ((r & RS) == RS) ip_neg(v[r]);
+    if ((r & RS) == RS) ip_neg(v[pm(pi, r)]);
 }
 
 __device__ __forceinline__ void reg_neg_rs(double2 (&v)[kR], uint32_t rsel) {
@@ -839,16 +849,17 @@ __device__ __forceinline__ void reg_scale_rs(double2 (&v)[kR], double sc, uint32
 }
 
 template <int A, int B>
-__device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel, uint32_t tl, bool tok) {
+__device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel, uint32_t tl, bool tok,
+                                         uint64_t pi = kPermId) {
   const uint64_t mask = __builtin_amdgcn_ballot_w64(tok);
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & ((1 << A) | (1 << B))) continue;
     if (((uint32_t)r & rsel) != rsel) continue;
     if (tl == 0) {  // uniform
-      ip_swap2(v[r | (1 << A)], v[r | (1 << B)]);
+      ip_swap2(v[pm(pi, r | (1 << A))], v[pm(pi, r | (1 << B))]);
     } else {
-      ip_cswap2(v[r | (1 << A)], v[r | (1 << B)], mask);
+      ip_cswap2(v[pm(pi, r | (1 << A))], v[pm(pi, r | (1 << B))], mask);
     }
   }
 }
@@ -856,12 +867,14 @@ __device__ __forceinline__ void reg_swap(double2 (&v)[kR], uint32_t rsel, uint32
 // A, B are the register slots of the lower and the higher local bit (basis
 // order of the 4x4 matrix: index bit 0 = slot A).
 template <int A, int B, bool F>
-__device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u) {
+__device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u, uint64_t pi = kPermId) {
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     if (r & ((1 << A) | (1 << B))) continue;
-    const double2 w0 = v[r], w1 = v[r | (1 << A)], w2 = v[r | (1 << B)];
-    const double2 w3 = v[r | (1 << A) | (1 << B)];
+    const int i0 = pm(pi, r), i1 = pm(pi, r | (1 << A)), i2 = pm(pi, r | (1 << B));
+    const int i3 = pm(pi, r | (1 << A) | (1 << B));
+    const double2 w0 = v[i0], w1 = v[i1], w2 = v[i2];
+    const double2 w3 = v[i3];
     double2 o[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -871,10 +884,10 @@ __device__ __forceinline__ void reg_mat4(double2 (&v)[kR], const double* u) {
       acc = macc<F>(acc, make_double2(u[8 * s + 6], u[8 * s + 7]), w3);
       o[s] = acc;
     }
-    v[r] = o[0];
-    v[r | (1 << A)] = o[1];
-    v[r | (1 << B)] = o[2];
-    v[r | (1 << A) | (1 << B)] = o[3];
+    v[i0] = o[0];
+    v[i1] = o[1];
+    v[i2] = o[2];
+    v[i3] = o[3];
   }
 }
 
@@ -944,12 +957,12 @@ __device__ __forceinline__ constexpr int round_of(int r) {
 // tr | regl(N, r): tw / tr are the thread parts (register bits clear) on the
 // writing and the reading side; ow / orr the slot offsets (slot_offsets) of
 // the two layouts, pa / pb the positions of the slots A / B that key the
-// rounds.
+// rounds. Logical register r is physical register pm(pi, r) on both sides.
 template <int Q, int A, int B>
 __device__ __forceinline__ void xchg_off(double2 (&v)[kR], uint32_t pa, uint32_t pb,
                                          const uint32_t (&ow)[kRegBits],
                                          const uint32_t (&orr)[kRegBits], uint32_t tw,
-                                         uint32_t tr, Xchg& X, bool act) {
+                                         uint32_t tr, Xchg& X, bool act, uint64_t pi = kPermId) {
   const uint32_t bw = swz(squeeze<Q>(tw, pa, pb)) << 4, br = swz(squeeze<Q>(tr, pa, pb)) << 4;
   char* buf = reinterpret_cast<char*>(X.lds);
 #pragma unroll
@@ -959,14 +972,15 @@ __device__ __forceinline__ void xchg_off(double2 (&v)[kR], uint32_t pa, uint32_t
 #pragma unroll
       for (int r = 0; r < kR; ++r)
         if (round_of<Q, A, B>(r) == c)
-          *reinterpret_cast<double2*>(buf + (bw ^ reg_offset<Q, A, B>(ow, r))) = v[r];
+          *reinterpret_cast<double2*>(buf + (bw ^ reg_offset<Q, A, B>(ow, r))) = v[pm(pi, r)];
     }
     __syncthreads();
     if (act) {
 #pragma unroll
       for (int r = 0; r < kR; ++r)
         if (round_of<Q, A, B>(r) == c)
-          v[r] = *reinterpret_cast<const double2*>(buf + (br ^ reg_offset<Q, A, B>(orr, r)));
+          v[pm(pi, r)] =
+              *reinterpret_cast<const double2*>(buf + (br ^ reg_offset<Q, A, B>(orr, r)));
     }
   }
 }
@@ -985,7 +999,8 @@ __device__ __forceinline__ void xchg(double2 (&v)[kR], const RegLayout& L, const
 // kXchgOld, kXchgNew of its coefficients).
 template <int Q, int A, int B>
 __device__ __forceinline__ void xchg_op(double2 (&v)[kR], const RegLayout& L, const RegLayout& N,
-                                        const uint32_t* w, Xchg& X, bool act) {
+                                        const uint32_t* w, Xchg& X, bool act,
+                                        uint64_t pi = kPermId) {
   uint32_t ow[kRegBits], orr[kRegBits];
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j) {
@@ -993,7 +1008,7 @@ __device__ __forceinline__ void xchg_op(double2 (&v)[kR], const RegLayout& L, co
     orr[j] = w[kXchgNew + j];
   }
   xchg_off<Q, A, B>(v, __builtin_ctz(L.rb[A]), __builtin_ctz(L.rb[B]), ow, orr, L.thr_l,
-                    N.thr_l, X, act);
+                    N.thr_l, X, act, pi);
 }
 
 // Exchange keyed by the first q of slots a < b (unchanged by it); 2^q rounds,
