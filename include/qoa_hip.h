@@ -115,6 +115,15 @@ typedef struct qoa_gate {
                                 op-loop kernel with the same other flags.
                                 Compiling takes seconds per circuit
                                 structure; see qoa_specialize_info.        */
+#define QOA_FUSE_FACTOR 128u /* register-resident passes take a scalar factor
+                                out of uncontrolled H, RX, RY (m00 = m11)
+                                and RZ gates and apply the pass's product of
+                                those factors once (in one of its ops): H
+                                becomes sums and differences, RX/RY one
+                                fused multiply-add per component, RZ a phase
+                                on half the amplitudes. Results within a few
+                                ulp (tested <= 1e-12). Implies QOA_FUSE_FMA.
+                                Single-state fused runs only.               */
 #define QOA_CZ_REF_MASKS 1u  /* CZ/CPHASE use the integer c|t as the bit mask,
                                 as QuantumState::apply_cz does
                                 (src/runtime/quantum_state.rs:338, 356-361).

@@ -31,6 +31,7 @@ QOA_FUSE_LDS_ONLY = 2
 QOA_FUSE_FMA = 4
 QOA_FUSE_MERGE_DIAG = 16
 QOA_FUSE_SPECIALIZE = 64
+QOA_FUSE_FACTOR = 128
 QOA_BATCH_AUTOTUNE = 32
 QOA_BLOCK_NO_RENORM = 8
 

@@ -236,7 +236,7 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
   const FusedPlan& P = c->plan;
   const uint32_t k = P.k;
   // merged diagonal runs exist in the FMA instantiation only
-  const bool fma = (opt.flags & (QOA_FUSE_FMA | QOA_FUSE_MERGE_DIAG)) != 0;
+  const bool fma = (opt.flags & (QOA_FUSE_FMA | QOA_FUSE_MERGE_DIAG | QOA_FUSE_FACTOR)) != 0;
   bool spec = false;
   if (opt.flags & QOA_FUSE_SPECIALIZE) {
     spec = true;

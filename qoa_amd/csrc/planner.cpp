@@ -691,6 +691,97 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
   store_layout(&cur, k, swaps, out);
 }
 
+// ---- factored passes (QOA_FUSE_FACTOR) ----------------------------------------
+// An uncontrolled H is nf * [[1, 1], [1, -1]], an RY or RX (m00 = m11 = c) is
+// c * [[1, m01/c], [m10/c, 1]], an RZ on a register slot is m00 * diag(1,
+// m11/m00). Inside a pass these ops run without their scalars, and the
+// product z of the scalars multiplies the coefficients of one op kept in its
+// plain form (an RZ when an RZ was factored, z then being complex, else any
+// candidate): linear ops commute with a scalar, so the pass's result is
+// unchanged up to rounding. Without the scalars the amplitudes grow by 1/|z|
+// inside the pass; factors are taken while that stays below 2^kFactorGrowth.
+constexpr double kFactorGrowth = 64.0;
+
+inline bool uncontrolled(const TileOp& op) {
+  return op.tl == 0 && op.rsel == 0 && op.gmask == 0 && op.gval == 0;
+}
+
+// 0: not a candidate, 1: H, 2: RY-shaped, 3: RX-shaped, 4: RZ on a register slot
+int factor_kind(const TileOp& op) {
+  if (!uncontrolled(op)) return 0;
+  if (op.kind == TK_H && op.c[1] == 0.0 && op.c[0] != 0.0) return 1;
+  if (op.kind == TK_M && op.c[0] == op.c[6] && op.c[0] != 0.0) {
+    if (op.t2 == kM2Real) return 2;
+    if (op.t2 == kM2Rx) return 3;
+  }
+  if (op.kind == TK_RZ && op.rt != 0xFF && (op.c[0] != 0.0 || op.c[1] != 0.0)) return 4;
+  return 0;
+}
+
+inline void cmul_into(double* a, const double* z) {  // a *= z
+  const double r = a[0] * z[0] - a[1] * z[1], i = a[0] * z[1] + a[1] * z[0];
+  a[0] = r;
+  a[1] = i;
+}
+
+void factor_pass(std::vector<TileOp>* ops, size_t begin) {
+  TileOp* sink = nullptr;
+  for (size_t i = begin; i < ops->size() && !sink; ++i)
+    if (factor_kind((*ops)[i]) == 4) sink = &(*ops)[i];
+  for (size_t i = begin; i < ops->size() && !sink; ++i)
+    if (factor_kind((*ops)[i]) != 0) sink = &(*ops)[i];
+  if (!sink) return;
+  double z[2] = {1.0, 0.0}, growth = 0.0;
+  for (size_t i = begin; i < ops->size(); ++i) {
+    TileOp& op = (*ops)[i];
+    const int fk = factor_kind(op);
+    // a complex factor needs the RZ sink (there is one whenever fk == 4 occurs)
+    if (fk == 0 || &op == sink) continue;
+    const double f = fk == 4 ? 1.0 : std::fabs(op.c[0]);
+    if (growth - std::log2(f) > kFactorGrowth) continue;
+    growth -= std::log2(f);
+    if (fk == 1) {  // nf [[1, 1], [1, -1]]
+      z[0] *= op.c[0];
+      z[1] *= op.c[0];
+      op.c[0] = 1.0;
+    } else if (fk == 2 || fk == 3) {  // c [[1, m01 / c], [m10 / c, 1]]
+      const double c = op.c[0];
+      z[0] *= c;
+      z[1] *= c;
+      for (int w = 2; w < 6; ++w) op.c[w] /= c;
+      op.c[0] = op.c[6] = 1.0;
+      op.t2 = fk == 2 ? kM2RealU : kM2RxU;
+    } else {  // m00 diag(1, m11 / m00): a phase on the registers of slot rt
+      const double m00[2] = {op.c[0], op.c[1]};
+      const double d = m00[0] * m00[0] + m00[1] * m00[1];
+      const double inv[2] = {m00[0] / d, -m00[1] / d};
+      double ph[2] = {op.c[2], op.c[3]};
+      cmul_into(ph, inv);
+      cmul_into(z, m00);
+      op.kind = TK_DPHASE;
+      op.rsel = (uint8_t)(1u << op.rt);
+      op.lmask = 1u << op.t;
+      op.rt = 0xFF;
+      op.c[0] = ph[0];
+      op.c[1] = ph[1];
+      op.c[2] = op.c[3] = 0.0;
+    }
+  }
+  if (z[0] == 1.0 && z[1] == 0.0) return;
+  switch (sink->kind) {
+    case TK_RZ:
+      cmul_into(sink->c, z);
+      cmul_into(sink->c + 2, z);
+      break;
+    case TK_H:  // z is real here (no RZ was factored)
+      sink->c[0] *= z[0];
+      break;
+    default:  // TK_M of shape kM2Real / kM2Rx (z real): every stored part
+      for (int w = 0; w < 8; ++w) sink->c[w] *= z[0];
+      break;
+  }
+}
+
 // ---- merged diagonal runs (QOA_FUSE_MERGE_DIAG) -------------------------------
 // VALU instructions of a register op on the 16 amplitudes of a thread (a
 // complex product is 6, a select 2) and of a TK_DIAGN op; the op dispatch
@@ -716,9 +807,11 @@ void diag_terms(const TileOp& op, std::vector<DiagTerm>* out) {
     t.c[0] = op.c[0];
     t.c[1] = op.c[1];
     out->push_back(t);
-    // m11 * conj(m00): |m00| = 1
-    t.c[0] = op.c[2] * op.c[0] + op.c[3] * op.c[1];
-    t.c[1] = op.c[3] * op.c[0] - op.c[2] * op.c[1];
+    // m11 / m00 (= m11 * conj(m00) for a plain RZ; a factored pass's RZ
+    // carries the pass's scalar in both entries)
+    const double d = op.c[0] * op.c[0] + op.c[1] * op.c[1];
+    t.c[0] = (op.c[2] * op.c[0] + op.c[3] * op.c[1]) / d;
+    t.c[1] = (op.c[3] * op.c[0] - op.c[2] * op.c[1]) / d;
     if (op.rt != 0xFF) {
       t.rs = 1u << op.rt;
     } else {
@@ -901,6 +994,8 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
     }
     if (pp.reg_mode) {
       segment_pass(pass_ops, pass_gi, k, (opt.flags & kPlanSwaps) != 0, &plan->ops, &plan->order);
+      if ((opt.flags & QOA_FUSE_FACTOR) && !(opt.flags & kPlanNoise))
+        factor_pass(&plan->ops, pp.op_begin);
       if ((opt.flags & QOA_FUSE_MERGE_DIAG) && !(opt.flags & kPlanNoise))
         merge_diag_runs(&plan->ops, pp.op_begin, &plan->pool);
     } else {

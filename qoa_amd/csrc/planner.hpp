@@ -50,8 +50,11 @@ constexpr uint32_t kDiagTermDoubles = sizeof(DiagTerm) / sizeof(double);
 // new one (0 for the slots that key the rounds).
 constexpr int kXchgSorted = 0, kXchgOld = 1, kXchgNew = 5, kXchgWords = 9;
 
-// TK_M shapes (t2): general, real, or real diagonal with imaginary off-diagonal.
-constexpr uint8_t kM2General = 0, kM2Real = 1, kM2Rx = 2;
+// TK_M shapes (t2): general, real, or real diagonal with imaginary
+// off-diagonal; the last two with m00 = m11 = 1 as kM2RealU / kM2RxU (the
+// factored forms of RY / RX, QOA_FUSE_FACTOR), whose products by 1 are
+// skipped.
+constexpr uint8_t kM2General = 0, kM2Real = 1, kM2Rx = 2, kM2RealU = 3, kM2RxU = 4;
 
 // Register-resident passes: each thread holds kRegAmps amplitudes of the tile
 // whose local indices differ in kRegBits "register bits"; the other bits

@@ -113,10 +113,14 @@ std::string op_call(const TileOp& op, uint32_t i, uint32_t k, bool fma, uint64_t
       const char* kind;
       bool shaped = false;  // kMReal / kMRx: the general 2x2 code under controls
       if (op.kind == TK_H) {
-        kind = op.c[1] == 0.0 ? "kHReal" : "TK_H";
+        kind = op.c[1] != 0.0 ? "TK_H" : op.c[0] == 1.0 ? "kHRaw" : "kHReal";
       } else if (op.kind == TK_M) {
         shaped = op.t2 == kM2Real || op.t2 == kM2Rx;
-        kind = op.t2 == kM2Real ? "kMReal" : op.t2 == kM2Rx ? "kMRx" : "TK_M";
+        kind = op.t2 == kM2Real    ? "kMReal"
+               : op.t2 == kM2Rx    ? "kMRx"
+               : op.t2 == kM2RealU ? "kMRealU"
+               : op.t2 == kM2RxU   ? "kMRxU"
+                                   : "TK_M";
       } else {
         kind = op.kind == TK_X ? "TK_X" : "TK_Y";
       }

@@ -179,6 +179,9 @@ __device__ __forceinline__ void set_layout(RegLayout& L, uint32_t tid, int k, co
 constexpr int kHReal = 16;  // TK_H with a real factor: a' = (nf*Re, nf*Im)
 constexpr int kMReal = 17;  // TK_M of shape kM2Real
 constexpr int kMRx = 18;    // TK_M of shape kM2Rx
+constexpr int kHRaw = 19;   // TK_H with nf = 1: sums and differences only
+constexpr int kMRealU = 20; // TK_M of shape kM2RealU (m00 = m11 = 1, real)
+constexpr int kMRxU = 21;   // TK_M of shape kM2RxU (m00 = m11 = 1, m01, m10 imaginary)
 
 // Coefficients of a 2x2 op: c[0..3] = m00, m01, m10, m11 (TK_H: c[0] = nf).
 struct Coef4 {
@@ -315,6 +318,43 @@ __device__ __forceinline__ void ip_h_real2(double2& x0, double2& y0, double2& x1
 }
 #undef QOA_HREAL_TAIL
 #undef QOA_T_HREAL
+
+// Factored 2x2 forms: four independent instructions per pair, the new x in
+// fresh registers (no copy back), y in place. Each equals the corresponding
+// plain form with those coefficients exactly (products by 1 are exact).
+// x' = x + y, y' = x - y (ip_h_real with nf = 1)
+__device__ __forceinline__ void ip_h_raw(double2& x, double2& y) {
+  double na, nb;
+  asm("v_add_f64 %[na], %[a], %[c]\n\t"
+      "v_add_f64 %[nb], %[b], %[d]\n\t"
+      "v_add_f64 %[c], %[a], -%[c]\n\t"
+      "v_add_f64 %[d], %[b], -%[d]"
+      : [na] "=&v"(na), [nb] "=&v"(nb), [c] "+v"(y.x), [d] "+v"(y.y)
+      : [a] "v"(x.x), [b] "v"(x.y));
+  x = make_double2(na, nb);
+}
+// x' = x + q y, y' = r x + y, q, r real
+__device__ __forceinline__ void ip_mat2_real_u(double2& x, double2& y, double q, double r) {
+  double na, nb;
+  asm("v_fma_f64 %[na], %[q], %[c], %[a]\n\t"
+      "v_fma_f64 %[nb], %[q], %[d], %[b]\n\t"
+      "v_fma_f64 %[c], %[r], %[a], %[c]\n\t"
+      "v_fma_f64 %[d], %[r], %[b], %[d]"
+      : [na] "=&v"(na), [nb] "=&v"(nb), [c] "+v"(y.x), [d] "+v"(y.y)
+      : [a] "v"(x.x), [b] "v"(x.y), [q] "s"(q), [r] "s"(r));
+  x = make_double2(na, nb);
+}
+// x' = x + (i q) y, y' = (i r) x + y, q, r real
+__device__ __forceinline__ void ip_mat2_rx_u(double2& x, double2& y, double q, double r) {
+  double na, nb;
+  asm("v_fma_f64 %[na], -%[q], %[d], %[a]\n\t"
+      "v_fma_f64 %[nb], %[q], %[c], %[b]\n\t"
+      "v_fma_f64 %[c], -%[r], %[b], %[c]\n\t"
+      "v_fma_f64 %[d], %[r], %[a], %[d]"
+      : [na] "=&v"(na), [nb] "=&v"(nb), [c] "+v"(y.x), [d] "+v"(y.y)
+      : [a] "v"(x.x), [b] "v"(x.y), [q] "s"(q), [r] "s"(r));
+  x = make_double2(na, nb);
+}
 
 // x' = nf (x + y), y' = nf (x - y), nf complex and uniform: sums first, then
 // in-place products nf * s (the product's operand order is the per-gate sweep's)
@@ -610,6 +650,17 @@ __device__ __forceinline__ void pair_math(double2 x, double2 y, const Coef4& k, 
     const double nf = k.c[0].x;
     o0 = make_double2(nf * (x.x + y.x), nf * (x.y + y.y));
     o1 = make_double2(nf * (x.x - y.x), nf * (x.y - y.y));
+  } else if (KIND == kHRaw) {
+    o0 = make_double2(x.x + y.x, x.y + y.y);
+    o1 = make_double2(x.x - y.x, x.y - y.y);
+  } else if (KIND == kMRealU) {
+    const double q = k.c[1].x, r = k.c[2].x;
+    o0 = make_double2(__builtin_fma(q, y.x, x.x), __builtin_fma(q, y.y, x.y));
+    o1 = make_double2(__builtin_fma(r, x.x, y.x), __builtin_fma(r, x.y, y.y));
+  } else if (KIND == kMRxU) {
+    const double q = k.c[1].y, r = k.c[2].y;
+    o0 = make_double2(__builtin_fma(-q, y.y, x.x), __builtin_fma(q, y.x, x.y));
+    o1 = make_double2(__builtin_fma(-r, x.y, y.x), __builtin_fma(r, x.x, y.y));
   } else if (KIND == TK_H) {
     o0 = mulc<F>(k.c[0], cadd(x, y));
     o1 = mulc<F>(k.c[0], csub(x, y));
@@ -629,6 +680,9 @@ __device__ __forceinline__ void pair_math(double2 x, double2 y, const Coef4& k, 
 template <int KIND, bool F>
 __device__ __forceinline__ void pair_inplace(double2& x, double2& y, const Coef4& k) {
   if (KIND == kHReal) ip_h_real(x, y, k.c[0].x);
+  else if (KIND == kHRaw) ip_h_raw(x, y);
+  else if (KIND == kMRealU) ip_mat2_real_u(x, y, k.c[1].x, k.c[2].x);
+  else if (KIND == kMRxU) ip_mat2_rx_u(x, y, k.c[1].y, k.c[2].y);
   else if (KIND == TK_H) ip_h_cplx<F>(x, y, k.c[0]);
   else if (KIND == TK_M) {
     if (F) ip_mat2_fma(x, y, k);
@@ -1397,12 +1451,18 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const
 #pragma unroll
       for (int i = 0; i < 4; ++i) k.c[i] = op.c2(i);
       if (kind == TK_H) {
-        if (op.c(1) == 0.0) reg_pair_any<kHReal, F>(v, k, s0, rsel, tl, L.thr_l);
-        else reg_pair_any<TK_H, F>(v, k, s0, rsel, tl, L.thr_l);
+        if (op.c(1) == 0.0) {
+          if (op.c(0) == 1.0) reg_pair_any<kHRaw, F>(v, k, s0, rsel, tl, L.thr_l);
+          else reg_pair_any<kHReal, F>(v, k, s0, rsel, tl, L.thr_l);
+        } else {
+          reg_pair_any<TK_H, F>(v, k, s0, rsel, tl, L.thr_l);
+        }
       } else if (kind == TK_M) {
         const uint32_t shape = (op.w[0] >> 16) & 0xFFu;  // t2
         if (shape == kM2Real) reg_pair_any<kMReal, F>(v, k, s0, rsel, tl, L.thr_l);
         else if (shape == kM2Rx) reg_pair_any<kMRx, F>(v, k, s0, rsel, tl, L.thr_l);
+        else if (shape == kM2RealU) reg_pair_any<kMRealU, F>(v, k, s0, rsel, tl, L.thr_l);
+        else if (shape == kM2RxU) reg_pair_any<kMRxU, F>(v, k, s0, rsel, tl, L.thr_l);
         else reg_pair_any<TK_M, F>(v, k, s0, rsel, tl, L.thr_l);
       } else if (kind == TK_X) {
         reg_pair_any<TK_X, F>(v, k, s0, rsel, tl, L.thr_l);

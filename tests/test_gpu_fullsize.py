@@ -21,7 +21,7 @@ import pytest
 import pyoracle as po
 from qoa_amd import StateVector
 from qoa_amd import workloads as W
-from qoa_amd._native import QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_SPECIALIZE
+from qoa_amd._native import QOA_FUSE_FACTOR, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_SPECIALIZE
 from qoa_amd.batch import Batch
 
 pytestmark = pytest.mark.gpu
@@ -67,7 +67,10 @@ def test_c2_28_qubits_against_oracle():
         for name, flags in (("exact", 0), ("fma", QOA_FUSE_FMA),
                             ("merged diagonals", QOA_FUSE_MERGE_DIAG),
                             ("merged diagonals, specialised passes",
-                             QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE)):
+                             QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE),
+                            ("factored, merged diagonals", QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG),
+                            ("factored, merged diagonals, specialised passes",
+                             QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE)):
             sv.apply_reset_all()
             t1 = time.perf_counter()
             sv.run_fused(g, flags=flags)
@@ -77,9 +80,9 @@ def test_c2_28_qubits_against_oracle():
             say(f"fused {name}: max|d amp| {d:.3e}, 1-F {inf:.3e} "
                 f"({time.perf_counter() - t1:.1f} s with planning and compiling)")
             assert d <= 1e-12 and inf <= 1e-12, (name, d, inf)
-            if flags == QOA_FUSE_MERGE_DIAG:
+            if not flags & QOA_FUSE_SPECIALIZE:
                 merged = out
-            elif flags & QOA_FUSE_SPECIALIZE:  # the op-loop kernel's bits exactly
+            elif flags & QOA_FUSE_MERGE_DIAG:  # the op-loop kernel's bits exactly
                 assert np.array_equal(out, merged)
             del out
 

@@ -13,6 +13,7 @@ from qoa_amd._native import QOA_CZ_REF_MASKS
 QOA_FUSE_LDS_ONLY = 2
 QOA_FUSE_FMA = 4
 QOA_FUSE_MERGE_DIAG = 16
+QOA_FUSE_FACTOR = 128
 MODES = [0, QOA_FUSE_LDS_ONLY]  # register-resident segments (default) / LDS-resident
 from test_planner import mixed_circuit
 
@@ -97,10 +98,12 @@ def test_index64_paths_match():
     assert outs[0][2] == outs[1][2] and outs[0][3] == outs[1][3]
 
 
-@pytest.mark.parametrize("flags", [QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG])
+@pytest.mark.parametrize("flags", [QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_FACTOR,
+                                   QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG])
 @pytest.mark.parametrize("tile", [6, 8, 12])
 def test_fma_mode_within_tolerance(tile, flags):
-    """FMA products (and merged diagonal runs): not bit-identical, but every
+    """FMA products (and merged diagonal runs, factored passes): not
+    bit-identical, but every
     amplitude within 1e-12 of the oracle (original gate order) and fidelity
     >= 1 - 1e-12, on the layered circuit and on the full gate surface (2x2,
     4x4, controls)."""
@@ -119,7 +122,8 @@ def test_fma_mode_within_tolerance(tile, flags):
         assert fidelity(out, ref) >= 1 - 1e-12
 
 
-@pytest.mark.parametrize("mode", MODES + [QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG])
+@pytest.mark.parametrize("mode", MODES + [QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG,
+                                   QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG])
 def test_c2_28q_fused_vs_per_gate_full_size(mode):
     """The full 28-qubit, 3164-gate circuit: fused passes against per-gate
     sweeps on the device (size-independent checks: unit norm, agreement
@@ -156,5 +160,26 @@ def test_merged_diagonal_program_matches_emulator():
     with StateVector(n) as sv:
         sv.upload(a0)
         sv.run_fused(g, tile_qubits=tile, flags=QOA_FUSE_MERGE_DIAG)
+        out = sv.download()
+    assert np.max(np.abs(out - emu)) <= 1e-13
+
+
+def test_factored_program_matches_emulator():
+    """The factored program (H as sums and differences, RX/RY with unit
+    diagonals, RZ as half-register phases, one op per pass carrying the
+    scalars) on the device against the CPU emulation of the same program."""
+    from tile_emulator import plan_pool, plan_program, run_program
+
+    n, tile = 14, 12
+    flags = QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG
+    g = W.layered_random(n, 12, seed=19, ladders=True)
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=tile, flags=flags)
+    pool = plan_pool(n, g, tile_qubits=tile, flags=flags)
+    a0 = random_state(n, seed=4)
+    emu = a0.copy()
+    run_program(emu, n, ops, spos, rng, k, pool=pool)
+    with StateVector(n) as sv:
+        sv.upload(a0)
+        sv.run_fused(g, tile_qubits=tile, flags=flags)
         out = sv.download()
     assert np.max(np.abs(out - emu)) <= 1e-13

@@ -7,7 +7,7 @@ import pyoracle as po
 from kat_util import random_state
 from qoa_amd import plan_fused
 from qoa_amd import workloads as W
-from qoa_amd._native import GATE_DTYPE, OP, QOA_CZ_REF_MASKS, QOA_FUSE_MERGE_DIAG
+from qoa_amd._native import GATE_DTYPE, OP, QOA_CZ_REF_MASKS, QOA_FUSE_FACTOR, QOA_FUSE_MERGE_DIAG
 
 
 def mixed_circuit(n: int, count: int, seed: int) -> tuple[np.ndarray, np.ndarray]:
@@ -159,6 +159,61 @@ def test_merged_diagonal_runs_match_oracle(n, tile, ladders):
     orig = random_state(n, seed=n)
     po.run_gates(orig, g, matrices=mats)
     assert np.max(np.abs(a - orig)) <= 1e-12
+
+
+@pytest.mark.parametrize("n,tile,extra", [(13, 10, QOA_FUSE_MERGE_DIAG), (14, 12, 0), (12, 8, 0)])
+def test_factored_passes_match_oracle(n, tile, extra):
+    """QOA_FUSE_FACTOR: uncontrolled H, RX/RY and register-slot RZ lose their
+    scalar factor (H sums and differences, RX/RY with m00 = m11 = 1, RZ a
+    phase on the registers of its slot), and one op per pass takes the
+    product. The emulated program equals the oracle in the planned and the
+    original order within 1e-12; a lost or doubled factor (1/sqrt 2 of an H,
+    cos of a rotation, e^{-i theta/2} of an RZ) would not."""
+    from tile_emulator import TK_DPHASE, TK_H, TK_M, TK_RZ, plan_pool, plan_program, run_program
+
+    g = W.layered_random(n, 10, seed=n + 7, ladders=True)
+    flags = QOA_FUSE_FACTOR | extra
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=tile, flags=flags)
+    pool = plan_pool(n, g, tile_qubits=tile, flags=flags)
+    plain, _, _, _ = plan_program(n, g, tile_qubits=tile, flags=extra)
+    h_raw = (ops["kind"] == TK_H) & (ops["c"][:, 0] == 1.0) & (ops["c"][:, 1] == 0.0)
+    u = (ops["kind"] == TK_M) & np.isin(ops["t2"], (3, 4))
+    assert h_raw.sum() > 0 and u.sum() > 0
+    assert np.all(ops[u]["c"][:, 0] == 1.0) and np.all(ops[u]["c"][:, 6] == 1.0)
+    if not extra:  # RZ on register slots became phases on half the registers
+        assert (ops["kind"] == TK_RZ).sum() < (plain["kind"] == TK_RZ).sum()
+        assert (ops["kind"] == TK_DPHASE).sum() > (plain["kind"] == TK_DPHASE).sum()
+    order, _ = plan_fused(n, g, tile_qubits=tile, flags=flags)
+    ref = random_state(n, seed=n)
+    a = ref.copy()
+    po.run_gates(ref, g, order=order)
+    run_program(a, n, ops, spos, rng, k, pool=pool)
+    assert np.max(np.abs(a - ref)) <= 1e-12
+    orig = random_state(n, seed=n)
+    po.run_gates(orig, g)
+    assert np.max(np.abs(a - orig)) <= 1e-12
+
+
+def test_factored_growth_stays_bounded():
+    """Rotations near pi have tiny cosines: taking their factor out multiplies
+    the amplitudes by 1/|cos| inside the pass. The planner stops taking
+    factors before the product passes 2^64, so forty RY(pi - 1e-9) on one
+    qubit stay finite and within 1e-12 of the oracle."""
+    from tile_emulator import TK_M, plan_pool, plan_program, run_program
+
+    n = 10
+    rows = [(OP["H"], q, 0, 0, 0.0, 0) for q in range(n)]
+    rows += [(OP["RY"], 3, 0, 0, np.pi - 1e-9, 0)] * 40
+    rows += [(OP["RX"], 4, 0, 0, np.pi, 0)] * 10
+    g = np.array(rows, dtype=GATE_DTYPE)
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=8, flags=QOA_FUSE_FACTOR)
+    factored = (ops["kind"] == TK_M) & np.isin(ops["t2"], (3, 4))
+    assert 0 < factored.sum() < 50
+    a = random_state(n, seed=5)
+    ref = a.copy()
+    run_program(a, n, ops, spos, rng, k, pool=plan_pool(n, g, tile_qubits=8, flags=QOA_FUSE_FACTOR))
+    po.run_gates(ref, g)
+    assert np.all(np.isfinite(a)) and np.max(np.abs(a - ref)) <= 1e-12
 
 
 def test_sincos_coefficients_match_oracle():

@@ -34,7 +34,7 @@ def _private_code_cache(tmp_path, monkeypatch):
     monkeypatch.setenv("QOA_SPEC_CACHE", str(tmp_path / "spec"))
 
 
-@pytest.mark.parametrize("flags", [0, N.QOA_FUSE_MERGE_DIAG])
+@pytest.mark.parametrize("flags", [0, N.QOA_FUSE_MERGE_DIAG, N.QOA_FUSE_FACTOR | N.QOA_FUSE_MERGE_DIAG])
 def test_layered_circuit_compiles(flags, tmp_path):
     g = W.layered_random(14, 6, seed=3 + flags, ladders=True)
     before = _info()
