@@ -46,10 +46,17 @@ FP64_TFLOPS = 78.6
 SIMDS = 1024          # 256 CUs x 4 SIMDs; a wave64 VALU or SALU instruction issues in 4 cycles
 CLOCK_MHZ = 2400.0    # peak engine clock
 QOA_FUSE_LDS_ONLY, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_SPECIALIZE = 2, 4, 16, 64
-QOA_FUSE_FACTOR = 128
+QOA_FUSE_FACTOR, QOA_FUSE_RELABEL = 128, 256
 
 
 METHOD_NOTES = {
+    "fused-relabel": "fused register passes (12-qubit tiles) as kernels compiled for this "
+                     "circuit (hipRTC, before the timed steps; extras.c2_specialize), FP64 with "
+                     "fused multiply-adds, runs of diagonal gates merged per register group, "
+                     "scalar factors of H/RX/RY/RZ taken out and applied once per pass, qubits "
+                     "relabelled between passes (each pass stores the next pass's qubits at the "
+                     "contiguous low positions; the order is restored before the run returns): "
+                     "within 1e-12 of the per-gate FP64 sweeps (accuracy field)",
     "fused-factor": "fused register passes (12-qubit tiles) as kernels compiled for this "
                     "circuit (hipRTC, before the timed steps; extras.c2_specialize), FP64 with "
                     "fused multiply-adds, runs of diagonal gates merged per register group, "
@@ -254,6 +261,9 @@ def run_circuit(sv, gates, method: str, steps: int, warmup: int, tag: str):
             sv.run_fused(gates, flags=QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE)
         elif method == "fused-factor":
             sv.run_fused(gates, flags=QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE)
+        elif method == "fused-relabel":
+            sv.run_fused(gates, flags=QOA_FUSE_RELABEL | QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG
+                         | QOA_FUSE_SPECIALIZE)
         else:
             sv.run_fused(gates)
 
@@ -637,9 +647,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--method", default="fused-factor",
+    ap.add_argument("--method", default="fused-relabel",
                     choices=["fused", "fused-fma", "fused-merge", "fused-spec", "fused-factor",
-                             "fused-lds", "per-gate"])
+                             "fused-relabel", "fused-lds", "per-gate"])
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=20241203)
@@ -697,7 +707,8 @@ def main() -> int:
         return want is None or label in want
 
     extras = {}
-    if spec1["compiles"] > spec0["compiles"] or args.method in ("fused-spec", "fused-factor"):
+    if spec1["compiles"] > spec0["compiles"] or args.method in ("fused-spec", "fused-factor",
+                                                               "fused-relabel"):
         extras["c2_specialize"] = {
             "programs_compiled": spec1["compiles"] - spec0["compiles"],
             "compile_s": spec1["compile_seconds"] - spec0["compile_seconds"],
@@ -714,7 +725,7 @@ def main() -> int:
         extras["c2_per-gate"] = {"ms_per_step": el * 1e3, "amp_updates_per_s": amp_updates / el,
                                  "kernels": kk}
         for other in ("fused", "fused-fma", "fused-merge", "fused-spec", "fused-factor",
-                      "fused-lds"):
+                      "fused-relabel", "fused-lds"):
             el, kk = run_circuit(sv, gates, other, 1, 0, f"c2 {other}")
             d, f = sv.compare(ref)
             entry = {"ms_per_step": el * 1e3, "amp_updates_per_s": amp_updates / el,
@@ -831,7 +842,8 @@ def roofline(dom, args, copy=None) -> dict:
     key = {"fused": "fused_pass/28q-d40-ladders",
            "fused-merge": "fused_pass_merge/28q-d40-ladders",
            "fused-spec": "fused_pass_spec/28q-d40-ladders",
-           "fused-factor": "fused_pass_factor/28q-d40-ladders"}.get(args.method)
+           "fused-factor": "fused_pass_factor/28q-d40-ladders",
+           "fused-relabel": "fused_pass_relabel/28q-d40-ladders"}.get(args.method)
     prof = measured_profile(key) if (default and kind == "fused_pass" and key) else None
     if prof:
         out["traffic"] = prof["bytes_per_launch"]

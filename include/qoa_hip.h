@@ -124,6 +124,16 @@ typedef struct qoa_gate {
                                 on half the amplitudes. Results within a few
                                 ulp (tested <= 1e-12). Implies QOA_FUSE_FMA.
                                 Single-state fused runs only.               */
+#define QOA_FUSE_RELABEL 256u /* register-resident passes may store their
+                                tile with its qubits at other positions of
+                                the tile: the state's qubit order changes
+                                from pass to pass (the order is put back
+                                before the run returns), so each pass can
+                                bring the qubits the next pass needs to the
+                                low (contiguous) positions instead of always
+                                keeping qubits 0..2 in every tile. Fewer
+                                passes; data movement only (results as
+                                without it). Single-state fused runs only.  */
 #define QOA_CZ_REF_MASKS 1u  /* CZ/CPHASE use the integer c|t as the bit mask,
                                 as QuantumState::apply_cz does
                                 (src/runtime/quantum_state.rs:338, 356-361).
@@ -292,6 +302,13 @@ int qoa_plan_program(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
                      const double* matrices, const qoa_fuse_opts* opts, void* ops_out,
                      uint32_t max_ops, uint8_t* pass_spos_out, uint32_t* pass_range_out,
                      uint32_t max_passes, uint32_t* n_ops, uint32_t* n_passes, uint32_t* tile_k);
+
+/* The same plan's store positions (host only): per pass the physical
+   position local bit j is stored to (16 bytes; the load positions of
+   qoa_plan_program unless the plan relabels qubits, QOA_FUSE_RELABEL). */
+int qoa_plan_store_positions(uint32_t n_qubits, const qoa_gate* gates, uint32_t n_gates,
+                             const double* matrices, const qoa_fuse_opts* opts,
+                             uint8_t* pass_spos_st_out, uint32_t max_passes, uint32_t* n_passes);
 
 /* The same plan's coefficient pool (host only): MAT4 matrices (32 doubles)
    and merged diagonal terms (6 doubles each: rs, tl as two 32-bit words,

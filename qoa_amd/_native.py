@@ -32,6 +32,7 @@ QOA_FUSE_FMA = 4
 QOA_FUSE_MERGE_DIAG = 16
 QOA_FUSE_SPECIALIZE = 64
 QOA_FUSE_FACTOR = 128
+QOA_FUSE_RELABEL = 256
 QOA_BATCH_AUTOTUNE = 32
 QOA_BLOCK_NO_RENORM = 8
 
@@ -129,6 +130,7 @@ _SIGS = {
                         I32),
     "qoa_state_compare": ([P, P, DP, DP], I32),
     "qoa_plan_pool": ([U32, P, U32, P, P, P, U32, C.POINTER(U32)], I32),
+    "qoa_plan_store_positions": ([U32, P, U32, P, P, P, U32, C.POINTER(U32)], I32),
     "qoa_specialize_compile": ([U32, P, U32, P, P, C.c_char_p], I32),
     "qoa_specialize_info": ([C.POINTER(U32), C.POINTER(U32), DP], I32),
     "qoa_plan_program": ([U32, P, U32, P, P, P, U32, P, P, U32, C.POINTER(U32), C.POINTER(U32),

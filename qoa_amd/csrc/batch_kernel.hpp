@@ -60,15 +60,11 @@ void k_batch_reg(double2* __restrict__ states,
     else hi = mid - 1;
   }
   const WorkItem& it = items[lo];
-  Geo geo;
+  GeoSame geo;
   geo.k = it.k;
 #pragma unroll
   for (int j = 0; j < 4; ++j) geo.spw[j] = it.spw[j];
-  I base = (I)(blockIdx.x - it.block_begin);
-  for (int j = 0; j < geo.k; ++j) {
-    const int b = (int)spos_at(geo, j);
-    base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
-  }
+  const I base = tile_base((I)(blockIdx.x - it.block_begin), geo);
   // Groups of 11+-qubit tiles (the only ones the swap-free kernel runs) have
   // exactly 2^(k-4) threads per block; one-wave blocks of smaller tiles have
   // idle lanes.

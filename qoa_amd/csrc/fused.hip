@@ -29,12 +29,9 @@ __global__ __launch_bounds__(kThreads) void k_fused(double2* __restrict__ a,
   extern __shared__ double2 lds[];
   const uint32_t T = 1u << geo.k;
   const uint32_t tid = threadIdx.x;
-  // tile base: the block index spread over the bits outside S
-  I base = (I)blockIdx.x;
-  for (int j = 0; j < geo.k; ++j) {
-    const int b = (int)spos_at(geo, j);
-    base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
-  }
+  // tile base: the block index spread over the bits outside S (LDS passes
+  // load and store at the same positions, spw)
+  const I base = tile_base((I)blockIdx.x, geo);
   for (uint32_t l = tid; l < T; l += kThreads) lds[l] = a[base | scatter<I>(l, geo)];
   __syncthreads();
 
@@ -154,11 +151,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                   const TileOp* __restrict__ ops, int nops,
                                                   const double* __restrict__ pool, Geo geo) {
   extern __shared__ double2 lds[];
-  I base = (I)blockIdx.x;
-  for (int j = 0; j < geo.k; ++j) {
-    const int b = (int)spos_at(geo, j);
-    base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
-  }
+  const I base = tile_base((I)blockIdx.x, geo);
   reg_tile<I, F, false, false, false>(a, base, geo, ops, nops, pool, lds, true, false, NoiseArgs{});
 }
 
@@ -193,6 +186,13 @@ void fused_cache_free(qoa_state* s) {
   if (c->d_pool) hipFree(c->d_pool);
   delete c;
   s->fused_cache = nullptr;
+}
+
+Geo pass_geo(const PassPlan& pp, uint32_t k) {
+  Geo g{};
+  g.k = (int)k;
+  pass_geo_words(pp, g.spw, g.stw, g.srt);
+  return g;
 }
 
 int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
@@ -267,9 +267,7 @@ int run_fused(qoa_state* s, const qoa_gate* g, uint32_t ng, const double* mats,
   });
   const uint64_t tiles = 1ULL << (s->n - k);
   for (const PassPlan& pp : P.passes) {
-    Geo geo{};
-    geo.k = (int)k;
-    std::memcpy(geo.spw, pp.spos, sizeof(geo.spw));
+    const Geo geo = pass_geo(pp, k);
     const int nops = (int)(pp.op_end - pp.op_begin);
     const TileOp* ops = c->d_ops + pp.op_begin;
     double2* a = (double2*)s->amps;

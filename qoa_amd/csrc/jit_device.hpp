@@ -25,6 +25,16 @@ struct SpecTile {
   Xchg X;
 };
 
+// The tile a block takes: the hardware hands consecutive blocks to the eight
+// XCDs in turn, so with a multiple of eight tiles block b takes tile
+// (b mod 8) * tiles/8 + b/8: each XCD (and its L2) works through one
+// contiguous eighth of the tiles in order (measured 2 % faster per pass on
+// the 28-qubit layered circuit than tile b).
+__device__ __forceinline__ uint32_t spec_tile_index() {
+  const uint32_t b = blockIdx.x, g = gridDim.x;
+  return (g & 7u) ? b : (b & 7u) * (g >> 3) + (b >> 3);
+}
+
 // Loads the tile of block `tile` in the initial layout (the top kRegBits
 // local bits in registers), as reg_tile does.
 template <typename I>
@@ -34,11 +44,7 @@ __device__ __forceinline__ void spec_load(SpecTile<I>& S, double2* __restrict__ 
   S.k = k;
   S.tid = threadIdx.x;
   S.X.lds = lds;
-  I base = tile;
-  for (int j = 0; j < k; ++j) {
-    const int b = (int)spos_at(geo, j);
-    base = ((base >> b) << (b + 1)) | (base & ((I(1) << b) - I(1)));
-  }
+  const I base = tile_base(tile, geo);
   S.base = base;
 #pragma unroll
   for (int j = 0; j < kRegBits; ++j) S.rpos[j] = (uint8_t)(k - kRegBits + j);
@@ -71,10 +77,10 @@ __device__ __forceinline__ void spec_store(SpecTile<I>& S, double2* __restrict__
   uint32_t thr = S.L.thr_l;
   asm volatile("" : "+v"(thr));
   I gthr = 0;
-  for (int j = 0; j < k; ++j) gthr |= (I)((thr >> j) & 1u) << spos_at(geo, j);
+  for (int j = 0; j < k; ++j) gthr |= (I)((thr >> j) & 1u) << stpos_at(geo, j);
   I gb[kRegBits];
 #pragma unroll
-  for (int j = 0; j < kRegBits; ++j) gb[j] = (I)1 << spos_at(geo, S.rpos[j]);
+  for (int j = 0; j < kRegBits; ++j) gb[j] = (I)1 << stpos_at(geo, S.rpos[j]);
 #pragma unroll
   for (int r = 0; r < kR; ++r) {
     I o = S.base | gthr;

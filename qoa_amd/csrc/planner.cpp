@@ -227,8 +227,19 @@ uint8_t m2_shape(const double* c) {
   return kM2General;
 }
 
+// Physical positions of the logical qubits in mask m (pos: logical qubit ->
+// physical position; nullptr: the identity).
+inline uint64_t phys_mask(uint64_t m, const uint8_t* pos) {
+  if (!pos) return m;
+  uint64_t r = 0;
+  for (; m; m &= m - 1) r |= bit(pos[__builtin_ctzll(m)]);
+  return r;
+}
+
+// spos: the tile's logical qubits in local bit order; tile conditions on
+// qubits outside the tile are physical masks (through pos).
 void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<TileOp>* out,
-          std::vector<double>* pool, bool noise) {
+          std::vector<double>* pool, bool noise, const uint8_t* pos = nullptr) {
   TileOp op{};
   op.gi = f.gi;
   if (noise) {
@@ -245,8 +256,8 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
   }
   const uint64_t pin = f.pmask & S, pout = f.pmask & ~S;
   op.lmask = local_bits(pin, spos, k);
-  op.gmask = pout;
-  op.gval = pout;
+  op.gmask = phys_mask(pout, pos);
+  op.gval = op.gmask;
   std::copy(f.c, f.c + 8, op.c);
   switch (f.kind) {
     case FK_H: case FK_M: case FK_X: case FK_Y:
@@ -269,9 +280,9 @@ void emit(const FOp& f, uint64_t S, const uint8_t* spos, int k, std::vector<Tile
         o0.kind = o1.kind = TK_DPHASE;
         o0.lmask = o1.lmask = 0;
         o0.nins = o1.nins = 0;
-        o0.gmask = o1.gmask = bit(f.rzq);
+        o0.gmask = o1.gmask = phys_mask(bit(f.rzq), pos);
         o0.gval = 0;
-        o1.gval = bit(f.rzq);
+        o1.gval = o0.gmask;
         o1.c[0] = f.c[2];
         o1.c[1] = f.c[3];
         o0.nq = 0;  // the gate's noise is drawn once, after the second op
@@ -591,17 +602,24 @@ void place_pair(RegSlots* cur, uint32_t lo, uint32_t hi, bool swaps, uint32_t q,
   emit_swaps(*cur, &sl, out);
 }
 
-// The pass stores from its final layout: the lowest local bits (up to three)
-// must then sit on thread index bits below three (any order), so eight
-// consecutive lanes write 128 contiguous bytes.
-void store_layout(RegSlots* cur, uint32_t k, bool swaps, std::vector<TileOp>* out) {
+// The pass stores from its final layout: the local bits wst stored at the
+// lowest physical positions (the lowest local bits, up to three, unless the
+// pass relabels qubits) must then sit on thread index bits below three, so
+// eight consecutive lanes write 128 contiguous bytes.
+void store_layout(RegSlots* cur, uint32_t k, bool swaps, std::vector<TileOp>* out,
+                  uint32_t wst) {
   const uint32_t nlow = std::min<uint32_t>(3u, k - kRegBits);
-  if (!use_swaps(k, swaps)) {  // exchanges leave the thread bits in ascending order:
-    const uint32_t low = (1u << nlow) - 1u;  // keep the low bits out of the registers
-    if (cur->set & low) {
-      uint32_t target = cur->set & ~low;
+  if (!use_swaps(k, swaps)) {
+    // Exchanges leave the thread bits in ascending order: wst out of the
+    // registers, and every local bit below wst's highest that is not in wst
+    // in them (at most nlow such bits).
+    const uint32_t below = wst ? ((2u << (31 - __builtin_clz(wst))) - 1u) & ~wst : 0u;
+    if ((cur->set & wst) || (below & ~cur->set)) {
+      uint32_t target = below;
+      for (int j = 0; j < kRegBits && __builtin_popcount(target) < kRegBits; ++j)
+        if (!(wst & (1u << cur->s[j]))) target |= 1u << cur->s[j];
       for (int b = (int)k - 1; b >= 0 && __builtin_popcount(target) < kRegBits; --b)
-        if (!(low & (1u << b))) target |= 1u << b;
+        if (!(wst & (1u << b))) target |= 1u << b;
       exchange_layout(cur, target, k, swaps, out);
     }
     return;
@@ -635,8 +653,10 @@ void reg_fields(TileOp* op, const RegSlots& L) {
   if (op->kind == TK_RZ && (L.set & (1u << op->t))) op->rt = slot_of(L, op->t);
 }
 
+// wst: the local bits the store puts at the lowest physical positions.
 void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& gis, uint32_t k,
-                  bool swaps, std::vector<TileOp>* out, std::vector<uint32_t>* order) {
+                  bool swaps, std::vector<TileOp>* out, std::vector<uint32_t>* order,
+                  uint32_t wst) {
   const uint32_t L0 = ((1u << kRegBits) - 1u) << (k - kRegBits);
   const uint32_t allb = (1u << k) - 1u;
   std::vector<SegOp> m(ops.size());
@@ -688,7 +708,7 @@ void segment_pass(const std::vector<TileOp>& ops, const std::vector<uint32_t>& g
     }
     rem.swap(next);
   }
-  store_layout(&cur, k, swaps, out);
+  store_layout(&cur, k, swaps, out, wst);
 }
 
 // ---- factored passes (QOA_FUSE_FACTOR) ----------------------------------------
@@ -889,6 +909,53 @@ int canonical_gate(uint32_t n, const qoa_gate& g, const double* mats, uint32_t f
   return canonical(n, g, mats, flags, out, gi);
 }
 
+namespace {
+
+// The pass's tile qubits: from `seed`, greedily the qubit (or, when no single
+// one helps, the pair) that lets the most ops of `rem` run, up to k qubits.
+uint64_t choose_set(const std::vector<FOp>& ops, const std::vector<uint32_t>& rem, uint64_t seed,
+                    uint32_t n, uint32_t k, uint64_t allq) {
+  uint64_t S = seed;
+  int cur = simulate(ops, rem, S, allq);
+  while ((uint32_t)__builtin_popcountll(S) < k) {
+    int best = -1, bests = -1;
+    for (uint32_t q = 0; q < n; ++q) {
+      if (S & bit(q)) continue;
+      const int sc = simulate(ops, rem, S | bit(q), allq);
+      if (sc > bests) {
+        bests = sc;
+        best = (int)q;
+      }
+    }
+    if (bests <= cur && (uint32_t)__builtin_popcountll(S) + 2 <= k) {
+      // no single qubit helps: try the best pair (two-qubit gates need both)
+      int b1 = -1, b2 = -1, bp = cur;
+      for (uint32_t q = 0; q < n; ++q) {
+        if (S & bit(q)) continue;
+        for (uint32_t r = q + 1; r < n; ++r) {
+          if (S & bit(r)) continue;
+          const int sc = simulate(ops, rem, S | bit(q) | bit(r), allq);
+          if (sc > bp) {
+            bp = sc;
+            b1 = (int)q;
+            b2 = (int)r;
+          }
+        }
+      }
+      if (b1 >= 0) {
+        S |= bit(b1) | bit(b2);
+        cur = bp;
+        continue;
+      }
+    }
+    S |= bit(best);
+    cur = bests;
+  }
+  return S;
+}
+
+}  // namespace
+
 int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
                const PlanOptions& opt, FusedPlan* plan) {
   if (n < 1 || n > 36) return set_error(QOA_ERR_ARG, "n_qubits %u out of range", n);
@@ -921,79 +988,101 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
   if (opt.flags & kPlanNoise)  // errors follow every gate: no gate passes another
     for (FOp& f : ops) f.mixb = f.all;
   const uint64_t allq = n >= 64 ? ~0ULL : ((1ULL << n) - 1);
-  const uint64_t low = m ? ((1ULL << m) - 1) : 0;
   std::vector<uint32_t> rem(ops.size());
   for (uint32_t i = 0; i < ops.size(); ++i) rem[i] = i;
+  const bool reg_ok = (k >= 8 || (opt.flags & kPlanForceReg)) && k >= (uint32_t)kRegBits &&
+                      !(opt.flags & QOA_FUSE_LDS_ONLY);
+  const bool relabel = (opt.flags & QOA_FUSE_RELABEL) && !(opt.flags & kPlanNoise) &&
+                       !(opt.flags & kPlanSwaps) && reg_ok && k < n && m > 0 &&
+                       k >= (uint32_t)kRegBits + 2 * m;
+  // logical qubit q sits at physical position pos[q]; inv is the inverse
+  uint8_t pos[64], inv[64];
+  for (uint32_t q = 0; q < 64; ++q) pos[q] = inv[q] = (uint8_t)q;
+  auto low_now = [&]() {  // logical qubits at the m lowest positions
+    uint64_t w = 0;
+    for (uint32_t p = 0; p < m; ++p) w |= bit(inv[p]);
+    return w;
+  };
 
-  while (!rem.empty()) {
-    uint64_t S;
-    if (k >= n) {
-      S = allq;
-    } else {
-      S = low | ops[rem[0]].active;
-      int cur = simulate(ops, rem, S, allq);
-      while ((uint32_t)__builtin_popcountll(S) < k) {
-        int best = -1, bests = -1;
-        for (uint32_t q = 0; q < n; ++q) {
-          if (S & bit(q)) continue;
-          const int sc = simulate(ops, rem, S | bit(q), allq);
-          if (sc > bests) {
-            bests = sc;
-            best = (int)q;
-          }
-        }
-        if (bests <= cur && (uint32_t)__builtin_popcountll(S) + 2 <= k) {
-          // no single qubit helps: try the best pair (two-qubit gates need both)
-          int b1 = -1, b2 = -1, bp = cur;
-          for (uint32_t q = 0; q < n; ++q) {
-            if (S & bit(q)) continue;
-            for (uint32_t r = q + 1; r < n; ++r) {
-              if (S & bit(r)) continue;
-              const int sc = simulate(ops, rem, S | bit(q) | bit(r), allq);
-              if (sc > bp) {
-                bp = sc;
-                b1 = (int)q;
-                b2 = (int)r;
-              }
-            }
-          }
-          if (b1 >= 0) {
-            S |= bit(b1) | bit(b2);
-            cur = bp;
-            continue;
-          }
-        }
-        S |= bit(best);
-        cur = bests;
-      }
-    }
+  // One pass over the tile qubits S (logical), running the ops `take` (in
+  // order); wn: the logical qubits the pass stores at the m lowest positions.
+  auto add_pass = [&](uint64_t S, const std::vector<uint32_t>& take, uint64_t wn) {
     PassPlan pp{};
     pp.k = k;
-    int j = 0;
-    for (uint32_t q = 0; q < n; ++q)
-      if (S & bit(q)) pp.spos[j++] = (uint8_t)q;
-    pp.reg_mode =
-        ((k >= 8 || (opt.flags & kPlanForceReg)) && k >= (uint32_t)kRegBits &&
-         !(opt.flags & QOA_FUSE_LDS_ONLY)) ? 1u : 0u;
+    pp.reg_mode = reg_ok ? 1u : 0u;
     pp.op_begin = (uint32_t)plan->ops.size();
-    Blocker blk;
-    std::vector<uint32_t> next;
-    next.reserve(rem.size());
+    uint8_t lq[16];  // logical qubit of local bit j
+    uint32_t wst = (1u << std::min<uint32_t>(3u, k - kRegBits)) - 1u;
+    uint8_t npos[64];
+    std::memcpy(npos, pos, sizeof(npos));
+    if (!relabel) {
+      int j = 0;
+      for (uint32_t q = 0; q < n; ++q)
+        if (S & bit(q)) lq[j++] = (uint8_t)q;
+    } else {
+      // local bits: the current low qubits (by position), then the other
+      // qubits of wn, then the rest of S by position
+      const uint64_t w = low_now();
+      int j = 0;
+      for (uint32_t p = 0; p < m; ++p) lq[j++] = inv[p];
+      for (uint32_t q = 0; q < n; ++q)
+        if ((wn & bit(q)) && !(w & bit(q))) lq[j++] = (uint8_t)q;
+      for (uint32_t p = 0; p < n; ++p)
+        if ((S & bit(inv[p])) && !((w | wn) & bit(inv[p]))) lq[j++] = inv[p];
+      // new positions: wn to 0..m-1 (lanes 0..m-1 store them; any order
+      // within the 128-byte run), its own home first; the others to their
+      // home position when it is in the tile, else where they are, else any
+      // free tile position
+      uint64_t freep = 0;  // free physical positions (of the tile's)
+      for (uint32_t i = 0; i < k; ++i) freep |= bit(pos[lq[i]]);
+      const uint64_t lowp = (1ULL << m) - 1;
+      wst = 0;
+      bool placed[16] = {};
+      for (uint32_t i = 0; i < k; ++i)
+        if ((wn & bit(lq[i])) && lq[i] < m) {
+          npos[lq[i]] = lq[i];
+          freep &= ~bit(lq[i]);
+          placed[i] = true;
+        }
+      for (uint32_t i = 0; i < k; ++i)
+        if (wn & bit(lq[i])) {
+          wst |= 1u << i;
+          if (placed[i]) continue;
+          const uint32_t p = (uint32_t)__builtin_ctzll(freep & lowp);
+          npos[lq[i]] = (uint8_t)p;
+          freep &= ~bit(p);
+          placed[i] = true;
+        }
+      for (int pass = 0; pass < 3; ++pass)
+        for (uint32_t i = 0; i < k; ++i) {
+          if (placed[i]) continue;
+          const uint32_t q = lq[i];
+          uint32_t want = 64;
+          if (pass == 0 && (freep & bit(q))) want = q;
+          if (pass == 1 && (freep & bit(pos[q]))) want = pos[q];
+          if (pass == 2) want = (uint32_t)__builtin_ctzll(freep);
+          if (want == 64) continue;
+          npos[q] = (uint8_t)want;
+          freep &= ~bit(want);
+          placed[i] = true;
+        }
+    }
+    for (uint32_t i = 0; i < k; ++i) {
+      pp.spos[i] = pos[lq[i]];
+      pp.spos_st[i] = npos[lq[i]];
+    }
     std::vector<TileOp> pass_ops;
     std::vector<uint32_t> pass_gi;
-    for (uint32_t i : rem) {
+    for (uint32_t i : take) {
       const FOp& f = ops[i];
-      if (blk.free(f.all, f.mixb) && (f.active & ~S) == 0) {
-        const size_t before = pass_ops.size();
-        emit(f, S, pp.spos, (int)k, &pass_ops, &plan->pool, (opt.flags & kPlanNoise) != 0);
-        for (size_t e = before; e < pass_ops.size(); ++e) pass_gi.push_back(f.gi);
-      } else {
-        blk.add(f.all, f.mixb);
-        next.push_back(i);
-      }
+      const size_t before = pass_ops.size();
+      emit(f, S, lq, (int)k, &pass_ops, &plan->pool, (opt.flags & kPlanNoise) != 0,
+           relabel ? pos : nullptr);
+      for (size_t e = before; e < pass_ops.size(); ++e) pass_gi.push_back(f.gi);
     }
     if (pp.reg_mode) {
-      segment_pass(pass_ops, pass_gi, k, (opt.flags & kPlanSwaps) != 0, &plan->ops, &plan->order);
+      segment_pass(pass_ops, pass_gi, k, (opt.flags & kPlanSwaps) != 0, &plan->ops, &plan->order,
+                   wst);
       if ((opt.flags & QOA_FUSE_FACTOR) && !(opt.flags & kPlanNoise))
         factor_pass(&plan->ops, pp.op_begin);
       if ((opt.flags & QOA_FUSE_MERGE_DIAG) && !(opt.flags & kPlanNoise))
@@ -1005,7 +1094,81 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
     }
     pp.op_end = (uint32_t)plan->ops.size();
     plan->passes.push_back(pp);
+    std::memcpy(pos, npos, sizeof(pos));
+    for (uint32_t q = 0; q < n; ++q) inv[pos[q]] = (uint8_t)q;
+  };
+
+  while (!rem.empty()) {
+    uint64_t S = k >= n ? allq : choose_set(ops, rem, low_now() | ops[rem[0]].active, n, k, allq);
+    if (relabel) {
+      // The last pass: when the qubits its ops mix, the low ones and every
+      // qubit away from its own position fit in one tile, take that tile, so
+      // the pass itself puts the order back (no extra pass).
+      Blocker b0;
+      bool last = true;
+      uint64_t need = low_now() | ((1ULL << m) - 1);
+      for (uint32_t i : rem) {
+        const FOp& f = ops[i];
+        if (!b0.free(f.all, f.mixb) || (f.active & ~S)) {
+          last = false;
+          break;
+        }
+        need |= f.active;
+      }
+      for (uint32_t q = 0; q < n; ++q)
+        if (pos[q] != q) need |= bit(q);
+      if (last && (uint32_t)__builtin_popcountll(need) <= k) {
+        for (uint32_t q = 0; q < n && (uint32_t)__builtin_popcountll(need) < k; ++q) need |= bit(q);
+        S = need;
+      }
+    }
+    Blocker blk;
+    std::vector<uint32_t> take, next;
+    next.reserve(rem.size());
+    for (uint32_t i : rem) {
+      const FOp& f = ops[i];
+      if (blk.free(f.all, f.mixb) && (f.active & ~S) == 0) {
+        take.push_back(i);
+      } else {
+        blk.add(f.all, f.mixb);
+        next.push_back(i);
+      }
+    }
+    uint64_t wn = low_now();
+    if (relabel) {
+      // the next pass's low qubits: qubits of S the next pass would choose
+      // anyway (its choice without the low constraint), the current low ones
+      // first; after the last pass the home qubits 0..m-1 where possible
+      const uint64_t home = (1ULL << m) - 1;
+      const uint64_t pref = next.empty() ? home : choose_set(ops, next, ops[next[0]].active, n, k, allq);
+      uint64_t cand = S & pref, w = 0;
+      for (uint64_t c = cand & wn; c && (uint32_t)__builtin_popcountll(w) < m; c &= c - 1)
+        w |= c & (~c + 1);
+      for (uint64_t c = cand & ~wn; c && (uint32_t)__builtin_popcountll(w) < m; c &= c - 1)
+        w |= c & (~c + 1);
+      for (uint64_t c = wn; c && (uint32_t)__builtin_popcountll(w) < m; c &= c - 1)
+        w |= c & (~c + 1);
+      wn = w;
+    }
+    add_pass(S, take, wn);
     rem.swap(next);
+  }
+  if (relabel) {  // put every qubit back at its own position
+    uint64_t mis = 0;
+    for (uint32_t q = 0; q < n; ++q)
+      if (pos[q] != q) mis |= bit(q);
+    if (mis) {
+      uint64_t S = mis | low_now() | ((1ULL << m) - 1);
+      for (uint32_t q = 0; q < n && (uint32_t)__builtin_popcountll(S) < k; ++q) S |= bit(q);
+      if ((uint32_t)__builtin_popcountll(S) > k) {  // too scattered: plan without relabelling
+        PlanOptions o2 = opt;
+        o2.flags &= ~QOA_FUSE_RELABEL;
+        return build_plan(n, g, ng, mats, o2, plan);
+      }
+      add_pass(S, {}, (1ULL << m) - 1);
+      for (uint32_t q = 0; q < n; ++q)
+        if (pos[q] != q) return set_error(QOA_ERR_UNSUPPORTED, "relabelled plan does not end in order");
+    }
   }
   return QOA_OK;
 }
@@ -1050,6 +1213,36 @@ extern "C" int qoa_plan_program(uint32_t n, const qoa_gate* g, uint32_t ng, cons
       range_out[3 * p + 2] = plan.passes[p].reg_mode;
     }
   }
+  return QOA_OK;
+}
+
+extern "C" int qoa_plan_store_positions(uint32_t n, const qoa_gate* g, uint32_t ng,
+                                        const double* mats, const qoa_fuse_opts* opts,
+                                        uint8_t* spos_st_out, uint32_t max_passes,
+                                        uint32_t* n_passes) {
+  using namespace qoa;
+  if (!g && ng) return set_error(QOA_ERR_ARG, "gates is NULL");
+  if (!n_passes) return set_error(QOA_ERR_ARG, "n_passes is NULL");
+  PlanOptions po{opts ? opts->tile_qubits : 0u, opts ? opts->low_qubits : 0u,
+                 opts ? opts->flags : 0u};
+  std::vector<double> zeros;
+  if (!mats) {
+    uint64_t maxaux = 0;
+    for (uint32_t i = 0; i < ng; ++i)
+      if (g[i].op == QOA_OP_U2 || g[i].op == QOA_OP_U4)
+        maxaux = std::max<uint64_t>(maxaux, g[i].aux + 1);
+    zeros.assign(32 * maxaux + 32, 0.0);
+    mats = zeros.data();
+  }
+  FusedPlan plan;
+  int rc = build_plan(n, g, ng, mats, po, &plan);
+  if (rc) return rc;
+  *n_passes = (uint32_t)plan.passes.size();
+  if (spos_st_out && max_passes < plan.passes.size())
+    return set_error(QOA_ERR_ARG, "%zu passes do not fit in %u", plan.passes.size(), max_passes);
+  if (spos_st_out)
+    for (size_t p = 0; p < plan.passes.size(); ++p)
+      std::memcpy(spos_st_out + 16 * p, plan.passes[p].spos_st, 16);
   return QOA_OK;
 }
 

@@ -123,9 +123,32 @@ static_assert(sizeof(TileOp) == 112, "TileOp layout");
 struct PassPlan {
   uint32_t k;                 // tile qubits
   uint32_t reg_mode;          // 1: register-resident kernel, 0: LDS-resident kernel
-  uint8_t spos[16];           // physical positions of the tile qubits, ascending
+  uint8_t spos[16];           // physical position local bit j is loaded from
+  uint8_t spos_st[16];        // and stored to (spos unless the pass relabels qubits)
   uint32_t op_begin, op_end;  // range in the flat op array
 };
+
+// A pass's geometry words (the kernels' Geo): load positions, store
+// positions, and the positions in ascending order, four bytes to a word.
+inline void pass_geo_words(const PassPlan& pp, uint32_t spw[4], uint32_t stw[4],
+                           uint32_t srt[4]) {
+  uint8_t s[16] = {};
+  for (uint32_t j = 0; j < pp.k && j < 16; ++j) s[j] = pp.spos[j];
+  for (uint32_t i = 1; i < pp.k && i < 16; ++i)  // insertion sort of k bytes
+    for (uint32_t j = i; j > 0 && s[j - 1] > s[j]; --j) {
+      const uint8_t t = s[j];
+      s[j] = s[j - 1];
+      s[j - 1] = t;
+    }
+  for (int w = 0; w < 4; ++w) {
+    spw[w] = stw[w] = srt[w] = 0;
+    for (int b = 0; b < 4; ++b) {
+      spw[w] |= (uint32_t)pp.spos[4 * w + b] << (8 * b);
+      stw[w] |= (uint32_t)pp.spos_st[4 * w + b] << (8 * b);
+      srt[w] |= (uint32_t)s[4 * w + b] << (8 * b);
+    }
+  }
+}
 
 struct FusedPlan {
   uint32_t n = 0, k = 0;

@@ -205,19 +205,22 @@ void pass_source(const FusedPlan& P, uint32_t p, bool fma, bool idx64, std::stri
   const uint32_t k = pp.k;
   const uint32_t threads = 1u << (k - kRegBits);
   const char* I = idx64 ? "uint64_t" : "uint32_t";
-  uint32_t spw[4] = {0, 0, 0, 0};
-  std::memcpy(spw, pp.spos, sizeof(spw));
-  char head[512];
+  uint32_t spw[4], stw[4], srt[4];
+  pass_geo_words(pp, spw, stw, srt);
+  char head[1024];
   std::snprintf(head, sizeof(head),
                 "extern \"C\" __global__ __launch_bounds__(%u) __attribute__((amdgpu_waves_per_eu(5)))"
                 " void qoa_spec_%u(double2* __restrict__ a, const TileOp* __restrict__ ops,"
                 " const double* __restrict__ pool) {\n"
                 "  extern __shared__ double2 lds[];\n"
-                "  Geo geo; geo.k = %u; geo.spw[0] = %uu; geo.spw[1] = %uu; geo.spw[2] = %uu;"
-                " geo.spw[3] = %uu;\n"
+                "  Geo geo; geo.k = %u;"
+                " geo.spw[0] = %uu; geo.spw[1] = %uu; geo.spw[2] = %uu; geo.spw[3] = %uu;"
+                " geo.stw[0] = %uu; geo.stw[1] = %uu; geo.stw[2] = %uu; geo.stw[3] = %uu;"
+                " geo.srt[0] = %uu; geo.srt[1] = %uu; geo.srt[2] = %uu; geo.srt[3] = %uu;\n"
                 "  SpecTile<%s> S;\n"
-                "  spec_load(S, a, geo, lds, (%s)blockIdx.x);\n",
-                threads, p, k, spw[0], spw[1], spw[2], spw[3], I, I);
+                "  spec_load(S, a, geo, lds, (%s)spec_tile_index());\n",
+                threads, p, k, spw[0], spw[1], spw[2], spw[3], stw[0], stw[1], stw[2], stw[3],
+                srt[0], srt[1], srt[2], srt[3], I, I);
   *out += head;
   Perm perm;
   for (uint32_t i = pp.op_begin; i < pp.op_end; ++i) {

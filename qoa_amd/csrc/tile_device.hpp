@@ -21,19 +21,49 @@ namespace tile {
 
 constexpr int kThreads = 256;
 
-// Tile geometry: k and the physical positions of the tile qubits, ascending,
-// four to a word (position j is byte j % 4 of spw[j / 4]). Positions are read
-// through spos_at, whose word selects keep them in scalar registers: a byte
-// array indexed by a loop variable becomes one vector memory load, and a wait
-// for it, per access (at the start of every tile, under the full HBM load).
+// Tile geometry: k and, four to a word (entry j is byte j % 4 of word j / 4),
+// the physical position local bit j is loaded from (spw) and stored to (stw),
+// and the same positions in ascending order (srt; the tile base inserts a
+// zero bit at each). Without relabelling, spw = stw = srt. Positions are read
+// through the *_at helpers, whose word selects keep them in scalar registers:
+// a byte array indexed by a loop variable becomes one vector memory load,
+// and a wait for it, per access (at the start of every tile, under the full
+// HBM load).
 struct Geo {
   int k;
   uint32_t spw[4];
+  uint32_t stw[4];
+  uint32_t srt[4];
 };
 
-__device__ __forceinline__ uint32_t spos_at(const Geo& g, uint32_t j) {
-  const uint32_t w = j < 4 ? g.spw[0] : j < 8 ? g.spw[1] : j < 12 ? g.spw[2] : g.spw[3];
+__device__ __forceinline__ uint32_t byte_at(const uint32_t (&w4)[4], uint32_t j) {
+  const uint32_t w = j < 4 ? w4[0] : j < 8 ? w4[1] : j < 12 ? w4[2] : w4[3];
   return (w >> (8 * (j & 3u))) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t spos_at(const Geo& g, uint32_t j) { return byte_at(g.spw, j); }
+__device__ __forceinline__ uint32_t stpos_at(const Geo& g, uint32_t j) { return byte_at(g.stw, j); }
+__device__ __forceinline__ uint32_t srt_at(const Geo& g, uint32_t j) { return byte_at(g.srt, j); }
+
+// The geometry of plans that never relabel (batches, serving): one set of
+// ascending positions for loads, stores and the tile base, in five words
+// (the batched kernels read them per work item and are short of registers).
+struct GeoSame {
+  int k;
+  uint32_t spw[4];
+};
+__device__ __forceinline__ uint32_t spos_at(const GeoSame& g, uint32_t j) { return byte_at(g.spw, j); }
+__device__ __forceinline__ uint32_t stpos_at(const GeoSame& g, uint32_t j) { return byte_at(g.spw, j); }
+__device__ __forceinline__ uint32_t srt_at(const GeoSame& g, uint32_t j) { return byte_at(g.spw, j); }
+
+// The tile base of tile index t: t's bits spread over the positions outside
+// the tile (a zero bit inserted at each tile position, ascending).
+template <typename I, typename G>
+__device__ __forceinline__ I tile_base(I t, const G& g) {
+  for (int j = 0; j < g.k; ++j) {
+    const int b = (int)srt_at(g, (uint32_t)j);
+    t = ((t >> b) << (b + 1)) | (t & ((I(1) << b) - I(1)));
+  }
+  return t;
 }
 
 __device__ __forceinline__ double2 cmul(double2 x, double2 y) {
@@ -60,8 +90,8 @@ __device__ __forceinline__ uint32_t dep(uint32_t x, const TileOp& op) {
   return x;
 }
 
-template <typename I>
-__device__ __forceinline__ I scatter(uint32_t l, const Geo& g) {
+template <typename I, typename G>
+__device__ __forceinline__ I scatter(uint32_t l, const G& g) {
   I off = 0;
   for (int j = 0; j < g.k; ++j) off |= (I)((l >> j) & 1u) << spos_at(g, j);
   return off;
@@ -1567,8 +1597,8 @@ __device__ __forceinline__ void reg_op(double2 (&v)[kR], const OpRegs& op, const
 // must then be 0). Waves without active threads skip the arithmetic. SWAPS:
 // the kernel takes layout ops that are swap lists (plans made with
 // kPlanSwaps); without it every layout op is a whole-tile exchange.
-template <typename I, bool F, bool NOISE, bool SCALAR_IDLE, bool SWAPS>
-__device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, const Geo& geo,
+template <typename I, bool F, bool NOISE, bool SCALAR_IDLE, bool SWAPS, typename G>
+__device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, const G& geo,
                                              const TileOp* __restrict__ ops, int nops,
                                              const double* __restrict__ pool, double2* lds,
                                              bool act, bool init_zero, const NoiseArgs& nz) {
@@ -1706,9 +1736,9 @@ __device__ __forceinline__ uint64_t reg_tile(double2* __restrict__ a, I base, co
   }
   if (act) {  // store from the final layout
     gthr = 0;
-    for (int j = 0; j < k; ++j) gthr |= (I)((L.thr_l >> j) & 1u) << spos_at(geo, j);
+    for (int j = 0; j < k; ++j) gthr |= (I)((L.thr_l >> j) & 1u) << stpos_at(geo, j);
 #pragma unroll
-    for (int j = 0; j < kRegBits; ++j) gb[j] = (I)1 << spos_at(geo, rpos[j]);
+    for (int j = 0; j < kRegBits; ++j) gb[j] = (I)1 << stpos_at(geo, rpos[j]);
 #pragma unroll
     for (int r = 0; r < kR; ++r) st_stream(a + gaddr(r), v[r]);
   }

@@ -21,7 +21,8 @@ import pytest
 import pyoracle as po
 from qoa_amd import StateVector
 from qoa_amd import workloads as W
-from qoa_amd._native import QOA_FUSE_FACTOR, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_SPECIALIZE
+from qoa_amd._native import (QOA_FUSE_FACTOR, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_RELABEL,
+                             QOA_FUSE_SPECIALIZE)
 from qoa_amd.batch import Batch
 
 pytestmark = pytest.mark.gpu
@@ -70,7 +71,12 @@ def test_c2_28_qubits_against_oracle():
                              QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE),
                             ("factored, merged diagonals", QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG),
                             ("factored, merged diagonals, specialised passes",
-                             QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE)):
+                             QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG | QOA_FUSE_SPECIALIZE),
+                            ("relabelled, factored, merged diagonals",
+                             QOA_FUSE_RELABEL | QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG),
+                            ("relabelled, factored, merged diagonals, specialised passes",
+                             QOA_FUSE_RELABEL | QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG
+                             | QOA_FUSE_SPECIALIZE)):
             sv.apply_reset_all()
             t1 = time.perf_counter()
             sv.run_fused(g, flags=flags)

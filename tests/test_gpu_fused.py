@@ -14,6 +14,7 @@ QOA_FUSE_LDS_ONLY = 2
 QOA_FUSE_FMA = 4
 QOA_FUSE_MERGE_DIAG = 16
 QOA_FUSE_FACTOR = 128
+QOA_FUSE_RELABEL = 256
 MODES = [0, QOA_FUSE_LDS_ONLY]  # register-resident segments (default) / LDS-resident
 from test_planner import mixed_circuit
 
@@ -123,7 +124,8 @@ def test_fma_mode_within_tolerance(tile, flags):
 
 
 @pytest.mark.parametrize("mode", MODES + [QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG,
-                                   QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG])
+                                   QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG,
+                                   QOA_FUSE_RELABEL | QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG])
 def test_c2_28q_fused_vs_per_gate_full_size(mode):
     """The full 28-qubit, 3164-gate circuit: fused passes against per-gate
     sweeps on the device (size-independent checks: unit norm, agreement
@@ -162,6 +164,30 @@ def test_merged_diagonal_program_matches_emulator():
         sv.run_fused(g, tile_qubits=tile, flags=QOA_FUSE_MERGE_DIAG)
         out = sv.download()
     assert np.max(np.abs(out - emu)) <= 1e-13
+
+
+@pytest.mark.parametrize("n,tile", [(16, 10), (17, 11), (18, 12)])
+def test_relabelled_program_matches_oracle_exactly(n, tile):
+    """Relabelling plans move qubits between positions from pass to pass and
+    put them back at the end: exact mode stays bit-identical to the oracle in
+    the planned order, and to the emulation of the same program."""
+    from tile_emulator import plan_program, plan_store_positions, run_program
+
+    g = W.layered_random(n, 16, seed=n + tile, ladders=True)
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=tile, flags=QOA_FUSE_RELABEL)
+    st = plan_store_positions(n, g, tile_qubits=tile, flags=QOA_FUSE_RELABEL)
+    assert np.any(st[:, :k] != spos[:, :k])
+    a0 = random_state(n, seed=7)
+    emu = a0.copy()
+    run_program(emu, n, ops, spos, rng, k, spos_st=st)
+    order, _ = plan_fused(n, g, tile_qubits=tile, flags=QOA_FUSE_RELABEL)
+    ref = a0.copy()
+    po.run_gates(ref, g, order=order)
+    with StateVector(n) as sv:
+        sv.upload(a0)
+        sv.run_fused(g, tile_qubits=tile, flags=QOA_FUSE_RELABEL)
+        out = sv.download()
+    assert np.array_equal(out, ref) and np.array_equal(emu, ref)
 
 
 def test_factored_program_matches_emulator():

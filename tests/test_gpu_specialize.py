@@ -13,7 +13,8 @@ from kat_util import random_state
 from qoa_amd import StateVector
 from qoa_amd import _native as N
 from qoa_amd import workloads as W
-from qoa_amd._native import QOA_FUSE_FACTOR, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_SPECIALIZE
+from qoa_amd._native import (QOA_FUSE_FACTOR, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_RELABEL,
+                             QOA_FUSE_SPECIALIZE)
 from test_planner import mixed_circuit
 
 pytestmark = pytest.mark.gpu
@@ -44,7 +45,8 @@ def _both(n, g, flags, tile=0, mats=None, a0=None):
 
 
 @pytest.mark.parametrize("flags", [0, QOA_FUSE_FMA, QOA_FUSE_MERGE_DIAG, QOA_FUSE_FACTOR,
-                                   QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG])
+                                   QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG, QOA_FUSE_RELABEL,
+                                   QOA_FUSE_RELABEL | QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG])
 @pytest.mark.parametrize("n,tile", [(16, 0), (18, 12), (15, 10)])
 def test_layered_circuit_equals_op_loop(n, tile, flags):
     g = W.layered_random(n, 12, seed=n + tile, ladders=True)
@@ -55,7 +57,7 @@ def test_layered_circuit_equals_op_loop(n, tile, flags):
     assert np.max(np.abs(spec - ref)) <= 1e-12
 
 
-@pytest.mark.parametrize("flags", [0, QOA_FUSE_FMA, QOA_FUSE_FACTOR])
+@pytest.mark.parametrize("flags", [0, QOA_FUSE_FMA, QOA_FUSE_FACTOR, QOA_FUSE_RELABEL])
 @pytest.mark.parametrize("n,tile", [(12, 8), (14, 12)])
 def test_mixed_surface_equals_op_loop(n, tile, flags):
     """U2/U4, Y, SWAP, CSWAP, Toffoli, CY, controls on register and thread

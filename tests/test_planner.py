@@ -5,6 +5,7 @@ import pytest
 
 import pyoracle as po
 from kat_util import random_state
+from qoa_amd import _native as N
 from qoa_amd import plan_fused
 from qoa_amd import workloads as W
 from qoa_amd._native import GATE_DTYPE, OP, QOA_CZ_REF_MASKS, QOA_FUSE_FACTOR, QOA_FUSE_MERGE_DIAG
@@ -214,6 +215,64 @@ def test_factored_growth_stays_bounded():
     run_program(a, n, ops, spos, rng, k, pool=plan_pool(n, g, tile_qubits=8, flags=QOA_FUSE_FACTOR))
     po.run_gates(ref, g)
     assert np.all(np.isfinite(a)) and np.max(np.abs(a - ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("n,tile,kind", [(14, 10, "layered"), (15, 12, "layered"),
+                                         (13, 10, "mixed"), (14, 12, "mixed"), (16, 12, "ladder")])
+def test_relabelled_plans_match_oracle(n, tile, kind):
+    """QOA_FUSE_RELABEL: passes store their tile with qubits at other tile
+    positions, so the next pass finds its own qubits at the three low
+    positions. Pure data movement: the emulated program (loads and stores at
+    the plan's positions) equals the oracle in the planned order bit for bit,
+    the state ends in the original qubit order, every register pass loads and
+    stores the three low positions from lanes 0..2, and passes do relabel."""
+    from tile_emulator import plan_program, plan_store_positions, run_program
+
+    mats = None
+    if kind == "mixed":  # U4 (MAT4) runs on the GPU only
+        g, mats = mixed_circuit(n, 400, seed=n + tile)
+        g = g[g["op"] != OP["U4"]]
+    else:
+        g = W.layered_random(n, 16, seed=n, ladders=kind == "ladder")
+    flags = N.QOA_FUSE_RELABEL
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=tile, flags=flags, matrices=mats)
+    st = plan_store_positions(n, g, tile_qubits=tile, flags=flags, matrices=mats)
+    assert st.shape == spos.shape
+    assert np.any(st[:, :k] != spos[:, :k]), "no pass relabels"
+    order, _ = plan_fused(n, g, tile_qubits=tile, flags=flags)
+    ref = random_state(n, seed=n + 1)
+    a = ref.copy()
+    po.run_gates(ref, g, matrices=mats, order=order)
+    run_program(a, n, ops, spos, rng, k, spos_st=st)
+    assert np.array_equal(a, ref)
+
+
+def test_relabelled_factored_merged_plan_matches_oracle():
+    from tile_emulator import plan_pool, plan_program, plan_store_positions, run_program
+
+    n, tile = 15, 12
+    g = W.layered_random(n, 20, seed=31, ladders=True)
+    flags = N.QOA_FUSE_RELABEL | QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG
+    ops, spos, rng, k = plan_program(n, g, tile_qubits=tile, flags=flags)
+    st = plan_store_positions(n, g, tile_qubits=tile, flags=flags)
+    pool = plan_pool(n, g, tile_qubits=tile, flags=flags)
+    a = random_state(n, seed=2)
+    ref = a.copy()
+    run_program(a, n, ops, spos, rng, k, pool=pool, spos_st=st)
+    po.run_gates(ref, g)
+    assert np.max(np.abs(a - ref)) <= 1e-12
+
+
+def test_relabelling_cuts_passes_of_the_28_qubit_circuit():
+    """The 28-qubit layered circuit (3164 gates): fewer passes when the low
+    positions follow the next pass's qubits (35 with qubits 0..2 in every
+    tile)."""
+    from tile_emulator import plan_program
+
+    g = W.layered_random(28, 40, seed=20241203, ladders=True)
+    base = plan_program(28, g, flags=QOA_FUSE_MERGE_DIAG)[2]
+    rel = plan_program(28, g, flags=QOA_FUSE_MERGE_DIAG | N.QOA_FUSE_RELABEL)[2]
+    assert len(rel) < len(base)
 
 
 def test_sincos_coefficients_match_oracle():

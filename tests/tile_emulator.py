@@ -49,6 +49,23 @@ def plan_program(n, gates, tile_qubits=0, low_qubits=0, flags=0, matrices=None):
     return ops, spos, rng, k.value
 
 
+def plan_store_positions(n, gates, tile_qubits=0, low_qubits=0, flags=0,
+                         matrices=None) -> np.ndarray:
+    """Per pass, the physical position each local bit is stored to."""
+    lib = N.load()
+    f = lib.qoa_plan_store_positions
+    f.argtypes, f.restype = N._SIGS["qoa_plan_store_positions"]
+    g = gates_array(gates)
+    m = None if matrices is None else np.ascontiguousarray(matrices, dtype=np.complex128)
+    opts = N.QoaFuseOpts(tile_qubits, low_qubits, flags, 0)
+    cnt = C.c_uint32()
+    mp = None if m is None else N.ptr(m)
+    N.check(f(n, N.ptr(g), len(g), mp, C.byref(opts), None, 0, C.byref(cnt)))
+    out = np.zeros((cnt.value, 16), dtype=np.uint8)
+    N.check(f(n, N.ptr(g), len(g), mp, C.byref(opts), N.ptr(out), cnt.value, C.byref(cnt)))
+    return out
+
+
 def plan_pool(n, gates, tile_qubits=0, low_qubits=0, flags=0, matrices=None) -> np.ndarray:
     """The plan's coefficient pool (MAT4 matrices, merged diagonal terms)."""
     lib = N.load()
@@ -90,24 +107,33 @@ def _check_reg_fields(op, kind: int, lm: int, layout) -> None:
 
 
 def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int, pool=None,
-                batched: bool = False) -> None:
+                batched: bool = False, spos_st=None) -> None:
     """Applies the program to the complex128 state `a` in place (`pool`: the
     plan's coefficient pool, needed for merged diagonal ops; `batched`: a
     plan for the batched kernels, whose 12-qubit exchanges keep one slot
-    instead of two)."""
+    instead of two; `spos_st`: the store positions of relabelling plans, whose
+    register passes must then store the three low positions from thread
+    index bits 0..2)."""
     re, im = a.real.copy(), a.imag.copy()
     T = 1 << k
     loc = np.arange(T)
     for p in range(len(rng)):
         pos = [int(x) for x in spos[p][:k]]
+        pos_st = pos if spos_st is None else [int(x) for x in spos_st[p][:k]]
+        assert sorted(pos_st) == sorted(pos), "a tile stores to its own positions"
         off = np.zeros(T, dtype=np.int64)
-        for j, b in enumerate(pos):
+        off_st = np.zeros(T, dtype=np.int64)
+        for j, (b, bs) in enumerate(zip(pos, pos_st)):
             off |= ((loc >> j) & 1) << b
+            off_st |= ((loc >> j) & 1) << bs
         b0, b1, reg_mode = (int(x) for x in rng[p])
+        if spos_st is not None and reg_mode:  # the loads: lanes 0..2 on positions 0..2
+            assert sorted(pos[:3]) == [0, 1, 2], "coalesced loads"
         for tile in range(1 << (n - k)):
             layout = list(range(k - 4, k))  # every tile starts in the initial layout
-            base = _deposit(tile, pos)
+            base = _deposit(tile, sorted(pos))
             idx = base | off
+            idx_st = base | off_st
             vr, vi = re[idx].copy(), im[idx].copy()
             for op in ops[b0:b1]:
                 kind = int(op["kind"])
@@ -228,6 +254,9 @@ def run_program(a: np.ndarray, n: int, ops, spos, rng, k: int, pool=None,
                     vr[l | m1], vi[l | m1] = ur, ui
                 else:
                     raise NotImplementedError("MAT4 is covered by the GPU tests")
-            re[idx], im[idx] = vr, vi
+            if spos_st is not None and reg_mode:  # the stores, from the final layout
+                lanes = [b for b in range(k) if b not in layout][:3]
+                assert sorted(pos_st[b] for b in lanes) == [0, 1, 2], "coalesced stores"
+            re[idx_st], im[idx_st] = vr, vi
     a.real[:] = re
     a.imag[:] = im
