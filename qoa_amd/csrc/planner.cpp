@@ -1153,21 +1153,37 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
     add_pass(S, take, wn);
     rem.swap(next);
   }
-  if (relabel) {  // put every qubit back at its own position
-    uint64_t mis = 0;
-    for (uint32_t q = 0; q < n; ++q)
-      if (pos[q] != q) mis |= bit(q);
-    if (mis) {
-      uint64_t S = mis | low_now() | ((1ULL << m) - 1);
-      for (uint32_t q = 0; q < n && (uint32_t)__builtin_popcountll(S) < k; ++q) S |= bit(q);
-      if ((uint32_t)__builtin_popcountll(S) > k) {  // too scattered: plan without relabelling
+  if (relabel) {
+    // Put every qubit back at its own position: passes without ops, each
+    // over the low qubits and whole cycles of the position permutation (a
+    // qubit reaches its home when the qubit there is in the same tile) as
+    // far as they fit, else the longest chain that does.
+    const uint64_t home_low = (1ULL << m) - 1;
+    for (int guard = 0;; ++guard) {
+      uint64_t mis = 0;
+      for (uint32_t q = 0; q < n; ++q)
+        if (pos[q] != q) mis |= bit(q);
+      if (!mis) break;
+      if (guard == 8) {  // not converging: plan without relabelling
         PlanOptions o2 = opt;
         o2.flags &= ~QOA_FUSE_RELABEL;
         return build_plan(n, g, ng, mats, o2, plan);
       }
-      add_pass(S, {}, (1ULL << m) - 1);
-      for (uint32_t q = 0; q < n; ++q)
-        if (pos[q] != q) return set_error(QOA_ERR_UNSUPPORTED, "relabelled plan does not end in order");
+      uint64_t S = low_now() | home_low;
+      for (uint64_t c = mis; c; c &= c - 1) {  // whole cycles
+        const uint32_t q0 = (uint32_t)__builtin_ctzll(c);
+        uint64_t cyc = 0;
+        for (uint32_t q = q0; !(cyc & bit(q)); q = inv[q]) cyc |= bit(q);
+        if ((uint32_t)__builtin_popcountll(S | cyc) <= k) S |= cyc;
+      }
+      for (uint64_t c = S & mis; c && (uint32_t)__builtin_popcountll(S) < k; c &= c - 1) {
+        // chains: the qubit sitting at a member's home joins while room is left
+        for (uint32_t q = (uint32_t)__builtin_ctzll(c);
+             (uint32_t)__builtin_popcountll(S) < k && !(S & bit(inv[q])); q = inv[q])
+          S |= bit(inv[q]);
+      }
+      for (uint32_t q = 0; q < n && (uint32_t)__builtin_popcountll(S) < k; ++q) S |= bit(q);
+      add_pass(S, {}, home_low);
     }
   }
   return QOA_OK;

@@ -630,46 +630,49 @@ __device__ __forceinline__ void ip_swap2(double2& x, double2& y) {
   ip_swap(x.y, y.y);
 }
 
-// x <-> y in the lanes of `mask` only: the swaps run with exec narrowed to
-// those lanes (and restored in the same statement), so no register beyond x
-// and y is written (selects would need a copy of every word).
+// x <-> y in the lanes of `mask` only: 64-bit moves through one temporary
+// with exec narrowed to those lanes (restored in the same statement), so no
+// register beyond x, y and the temporary is written and every value stays in
+// an aligned register pair.
 __device__ __forceinline__ void ip_cswap2(double2& x, double2& y, uint64_t mask) {
-  uint64_t ux = __builtin_bit_cast(uint64_t, x.x), uy = __builtin_bit_cast(uint64_t, y.x);
-  uint64_t vx = __builtin_bit_cast(uint64_t, x.y), vy = __builtin_bit_cast(uint64_t, y.y);
-  uint32_t a0 = (uint32_t)ux, a1 = (uint32_t)(ux >> 32), b0 = (uint32_t)uy, b1 = (uint32_t)(uy >> 32);
-  uint32_t c0 = (uint32_t)vx, c1 = (uint32_t)(vx >> 32), d0 = (uint32_t)vy, d1 = (uint32_t)(vy >> 32);
+  double t;
   uint64_t save;
   asm volatile(
       "s_and_saveexec_b64 %[s], %[m]\n\t"
-      "v_swap_b32 %[a0], %[b0]\n\t"
-      "v_swap_b32 %[a1], %[b1]\n\t"
-      "v_swap_b32 %[c0], %[d0]\n\t"
-      "v_swap_b32 %[c1], %[d1]\n\t"
+      "v_mov_b64 %[t], %[a]\n\t"
+      "v_mov_b64 %[a], %[c]\n\t"
+      "v_mov_b64 %[c], %[t]\n\t"
+      "v_mov_b64 %[t], %[b]\n\t"
+      "v_mov_b64 %[b], %[d]\n\t"
+      "v_mov_b64 %[d], %[t]\n\t"
       "s_mov_b64 exec, %[s]"
-      : [a0] "+v"(a0), [a1] "+v"(a1), [b0] "+v"(b0), [b1] "+v"(b1), [c0] "+v"(c0),
-        [c1] "+v"(c1), [d0] "+v"(d0), [d1] "+v"(d1), [s] "=&s"(save)
+      : [a] "+v"(x.x), [b] "+v"(x.y), [c] "+v"(y.x), [d] "+v"(y.y), [t] "=&v"(t), [s] "=&s"(save)
       : [m] "s"(mask)
       : "scc");
-  x = make_double2(__builtin_bit_cast(double, ((uint64_t)a1 << 32) | a0),
-                   __builtin_bit_cast(double, ((uint64_t)c1 << 32) | c0));
-  y = make_double2(__builtin_bit_cast(double, ((uint64_t)b1 << 32) | b0),
-                   __builtin_bit_cast(double, ((uint64_t)d1 << 32) | d0));
 }
 
-// x = lanes of `mask` ? n : x
+// x = lanes of `mask` ? n : x (exec-narrowed 64-bit moves)
 __device__ __forceinline__ void ip_select(double& x, double n, uint64_t mask) {
-  uint64_t ux = __builtin_bit_cast(uint64_t, x), un = __builtin_bit_cast(uint64_t, n);
-  uint32_t xl = (uint32_t)ux, xh = (uint32_t)(ux >> 32);
-  asm("v_cndmask_b32_e64 %[xl], %[xl], %[nl], %[m]\n\t"
-      "v_cndmask_b32_e64 %[xh], %[xh], %[nh], %[m]"
-      : [xl] "+v"(xl), [xh] "+v"(xh)
-      : [nl] "v"((uint32_t)un), [nh] "v"((uint32_t)(un >> 32)), [m] "s"(mask));
-  x = __builtin_bit_cast(double, ((uint64_t)xh << 32) | xl);
+  uint64_t save;
+  asm volatile(
+      "s_and_saveexec_b64 %[s], %[m]\n\t"
+      "v_mov_b64 %[x], %[n]\n\t"
+      "s_mov_b64 exec, %[s]"
+      : [x] "+v"(x), [s] "=&s"(save)
+      : [n] "v"(n), [m] "s"(mask)
+      : "scc");
 }
 
 __device__ __forceinline__ void ip_select2(double2& x, double2 n, uint64_t mask) {
-  ip_select(x.x, n.x, mask);
-  ip_select(x.y, n.y, mask);
+  uint64_t save;
+  asm volatile(
+      "s_and_saveexec_b64 %[s], %[m]\n\t"
+      "v_mov_b64 %[a], %[na]\n\t"
+      "v_mov_b64 %[b], %[nb]\n\t"
+      "s_mov_b64 exec, %[s]"
+      : [a] "+v"(x.x), [b] "+v"(x.y), [s] "=&s"(save)
+      : [na] "v"(n.x), [nb] "v"(n.y), [m] "s"(mask)
+      : "scc");
 }
 
 template <int KIND, bool F>
