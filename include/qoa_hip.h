@@ -122,8 +122,9 @@ typedef struct qoa_gate {
                                 becomes sums and differences, RX/RY one
                                 fused multiply-add per component, RZ a phase
                                 on half the amplitudes. Results within a few
-                                ulp (tested <= 1e-12). Implies QOA_FUSE_FMA.
-                                Single-state fused runs only.               */
+                                ulp (tested <= 1e-12). Implies QOA_FUSE_FMA
+                                for single states; batches (qoa_batch_create)
+                                take it for their noise-free jobs.          */
 #define QOA_FUSE_RELABEL 256u /* register-resident passes may store their
                                 tile with its qubits at other positions of
                                 the tile: the state's qubit order changes
@@ -398,7 +399,7 @@ typedef struct qoa_job_desc {
 } qoa_job_desc;
 /* Plans every job and allocates every instance's state in one device buffer
    (QOA_ERR_OOM when they do not fit). Gates use layer-1 semantics
-   (opts->flags may hold QOA_CZ_REF_MASKS, QOA_BATCH_AUTOTUNE).
+   (opts->flags may hold QOA_CZ_REF_MASKS, QOA_BATCH_AUTOTUNE, QOA_FUSE_FACTOR).
    QOA_C128 jobs (the default): fused FP64 register passes, tile =
    min(opts->tile_qubits or 12, n), trajectory noise allowed.
    QOA_C64 jobs (all jobs of a batch share one precision): the FP32-fused

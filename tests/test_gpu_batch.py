@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
+from qoa_amd import _native
 from qoa_amd import workloads as W
 from qoa_amd.batch import Batch
 from test_planner import mixed_circuit
@@ -37,6 +38,25 @@ def test_ansatz_batch_matches_oracle(tile):
         b.synchronize()
         for j in range(len(jobs)):
             assert np.array_equal(b.download(j), first[j])
+
+
+@pytest.mark.parametrize("tile", [8, 10, 12])
+def test_factored_ansatz_batch_within_tolerance(tile):
+    """QOA_FUSE_FACTOR for batches: RY as [[1, -t], [t, 1]] with the cosine
+    applied once per pass, RZ as a phase on half the registers; every job
+    within 1e-12 of the oracle (original order)."""
+    jobs = []
+    for j in range(24):
+        n, depth, g = W.ansatz_job(700 + j, n_min=4, n_max=15, d_min=2, d_max=12)
+        jobs.append((n, g))
+    with Batch(jobs, tile_qubits=tile, flags=_native.QOA_FUSE_FACTOR) as b:
+        b.run()
+        b.synchronize()
+        for j, (n, g) in enumerate(jobs):
+            out = b.download(j)
+            orig = po.state(n)
+            po.run_gates(orig, g)
+            assert np.max(np.abs(out - orig)) <= 1e-12, f"job {j} n={n}"
 
 
 def test_mixed_surface_batch_and_trajectory_copies():
