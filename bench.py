@@ -510,7 +510,8 @@ def extra_c5_rate(args, rate: float, n_jobs: int) -> dict:
     from qoa_amd.server import Server
 
     jobs, offsets = serving_stream(args, n_jobs, rate)
-    with Server(budget_bytes=int(200 * 2**30)) as srv:
+    # factored forms (the pass scalars commute with the trajectory errors)
+    with Server(budget_bytes=int(200 * 2**30), flags=QOA_FUSE_FACTOR) as srv:
         t_start = srv.now() + 0.2
         arrivals = t_start + offsets
         ids = [srv.submit(n, g, trajectories=64, p1=1e-3, p2=1e-2, noise_seed=args.seed + j,

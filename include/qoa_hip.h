@@ -124,7 +124,9 @@ typedef struct qoa_gate {
                                 on half the amplitudes. Results within a few
                                 ulp (tested <= 1e-12). Implies QOA_FUSE_FMA
                                 for single states; batches (qoa_batch_create)
-                                take it for their noise-free jobs.          */
+                                and the server (qoa_server_create) take it
+                                for their jobs, noisy ones included (the
+                                scalars commute with the Pauli errors).     */
 #define QOA_FUSE_RELABEL 256u /* register-resident passes may store their
                                 tile with its qubits at other positions of
                                 the tile: the state's qubit order changes
@@ -444,7 +446,7 @@ typedef struct qoa_server_opts {
   uint32_t max_items;       /* work items per step (0: 65536)                  */
   uint32_t max_ops;         /* op slots for all resident jobs (0: 2^21)         */
   uint32_t max_jobs;        /* resident jobs (0: 4096)                          */
-  uint32_t flags;           /* QOA_CZ_REF_MASKS                                 */
+  uint32_t flags;           /* QOA_CZ_REF_MASKS, QOA_FUSE_FACTOR                */
 } qoa_server_opts;
 typedef struct qoa_job_result {
   uint64_t job_id;

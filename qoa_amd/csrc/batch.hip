@@ -462,7 +462,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
           const uint32_t j = next.fetch_add(1);
           if (j >= n_jobs) break;
           const bool noisy = jobs[j].p1 > 0.0 || jobs[j].p2 > 0.0;
-          PlanOptions po{b->job_tile[j], opts ? opts->low_qubits : 0u,
+          PlanOptions po{b->job_tile[j], opts ? opts->low_qubits : 0u,  // noisy jobs too
                          (opts ? (opts->flags & (QOA_CZ_REF_MASKS | QOA_FUSE_FACTOR)) : 0u) |
                              kPlanForceReg | kPlanSwaps | (noisy ? kPlanNoise : 0u)};
           rcs[j] = build_plan(jobs[j].n_qubits, gates + jobs[j].gate_offset, jobs[j].n_gates,

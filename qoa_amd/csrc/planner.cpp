@@ -1083,7 +1083,7 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
     if (pp.reg_mode) {
       segment_pass(pass_ops, pass_gi, k, (opt.flags & kPlanSwaps) != 0, &plan->ops, &plan->order,
                    wst);
-      if ((opt.flags & QOA_FUSE_FACTOR) && !(opt.flags & kPlanNoise))
+      if (opt.flags & QOA_FUSE_FACTOR)  // scalars commute with the Pauli errors too
         factor_pass(&plan->ops, pp.op_begin);
       if ((opt.flags & QOA_FUSE_MERGE_DIAG) && !(opt.flags & kPlanNoise))
         merge_diag_runs(&plan->ops, pp.op_begin, &plan->pool);

@@ -192,7 +192,8 @@ void planner_main(qoa_server* s) {
     }
     const bool noisy = j->desc.p1 > 0.0 || j->desc.p2 > 0.0;
     PlanOptions po{std::min(s->opts.tile_qubits, j->desc.n_qubits), 0u,
-                   (s->opts.flags & QOA_CZ_REF_MASKS) | kPlanForceReg | kPlanSwaps | (noisy ? kPlanNoise : 0u)};
+                   (s->opts.flags & (QOA_CZ_REF_MASKS | QOA_FUSE_FACTOR)) | kPlanForceReg | kPlanSwaps |
+                       (noisy ? kPlanNoise : 0u)};
     const int rc = build_plan(j->desc.n_qubits, j->gates.data(), (uint32_t)j->gates.size(),
                               j->mats.empty() ? nullptr : j->mats.data(), po, &j->plan);
     {

@@ -103,6 +103,30 @@ def test_noisy_trajectories_match_oracle(tile):
             assert any(not np.array_equal(outs[0], o) for o in outs[1:]), "noise had no effect"
 
 
+@pytest.mark.parametrize("tile", [8, 12])
+def test_factored_noisy_trajectories_within_tolerance(tile):
+    """QOA_FUSE_FACTOR with trajectory noise: the pass scalars commute with the
+    Pauli errors, so every factored trajectory stays within 1e-12 of the
+    oracle trajectory with the same draws (original order)."""
+    p1, p2 = 0.05, 0.2
+    jobs = []
+    for j, n in enumerate([5, 9, 12, 14, 16]):
+        _, _, g = W.ansatz_job(500 + j, n_min=n, n_max=n, d_min=6, d_max=6)
+        jobs.append((n, g))
+    seeds = [2000 + j for j in range(len(jobs))]
+    T = 3
+    with Batch(jobs, tile_qubits=tile, trajectories=T, p1=p1, p2=p2, noise_seed=seeds,
+               flags=_native.QOA_FUSE_FACTOR) as b:
+        b.run()
+        b.synchronize()
+        for j, (n, g) in enumerate(jobs):
+            order = b.order(j)
+            for t in range(T):
+                ref = po.state(n)
+                po.run_trajectory(ref, g, seeds[j], t, p1, p2, order=order)
+                assert np.max(np.abs(b.download(j, t) - ref)) <= 1e-12, (j, n, t)
+
+
 def shared_table_jobs(sizes, count, seed0):
     """mixed_circuit jobs (every gate kind, U2/U4 included) with in-range
     operands, their matrices moved into one shared table."""
