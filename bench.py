@@ -409,7 +409,8 @@ def extra_c4(args) -> dict:
 
     pj = [(n, g) for n, _, g in jobs]
     t0 = time.perf_counter()
-    b = Batch(pj, tile_qubits=12, flags=QOA_BATCH_AUTOTUNE | QOA_FUSE_FACTOR)
+    fast = QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG
+    b = Batch(pj, tile_qubits=12, flags=QOA_BATCH_AUTOTUNE | fast)
     plan_s = time.perf_counter() - t0
     info = b.info()
     ms = timed_runs(b)
@@ -421,19 +422,20 @@ def extra_c4(args) -> dict:
     # the same batch with every kernel launched from the host (no graph)
     lib.qoa_set_option(b"batch_graphs", 0)
     try:
-        with Batch(pj, tile_qubits=12, flags=QOA_BATCH_AUTOTUNE | QOA_FUSE_FACTOR) as bd:
+        with Batch(pj, tile_qubits=12, flags=QOA_BATCH_AUTOTUNE | fast) as bd:
             ms_direct = timed_runs(bd)
     finally:
         lib.qoa_set_option(b"batch_graphs", 1)
-    with Batch(pj, tile_qubits=12, flags=QOA_FUSE_FACTOR) as b12:
+    with Batch(pj, tile_qubits=12, flags=fast) as b12:
         ms_12 = timed_runs(b12)
     amp_updates = float(sum(len(g) * (1 << n) for n, _, g in jobs))
     out = {"workload": "1024 hardware-efficient-ansatz circuits, n~U{8..24}, depth~U{10..60}, "
                        "FP64, all states resident", "jobs": 1024,
            "state_gib": info["total_amps"] * 16 / 2**30, "rounds": info["rounds"],
            "launches": "one graph replay per run (the rounds' kernels captured once)",
-           "arithmetic": "FP64 register passes with factored H/RX/RY/RZ (QOA_FUSE_FACTOR; "
-                         "within 1e-12, sampled jobs below)",
+           "arithmetic": "FP64 register passes with factored H/RX/RY/RZ and merged diagonal "
+                         "runs (QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG, FMA; within 1e-12, "
+                         "sampled jobs below); ms_per_batch_unfactored: exact arithmetic",
            "ms_per_batch": ms, "jobs_per_s": 1024 / (ms * 1e-3),
            "ms_per_batch_unfactored": ms_plain,
            "amp_updates_per_s": amp_updates / (ms * 1e-3),

@@ -104,8 +104,9 @@ typedef struct qoa_gate {
                                 into the amplitudes. Far fewer instructions
                                 for phase ladders; results within a few ulp
                                 (tested <= 1e-12) instead of bit-identical.
-                                Implies QOA_FUSE_FMA. Single-state fused
-                                runs only (batches and serving ignore it).  */
+                                Implies QOA_FUSE_FMA. Batches take it when
+                                none of their jobs is noisy; the server
+                                ignores it.                                 */
 #define QOA_FUSE_SPECIALIZE 64u /* register-resident passes run as kernels
                                 compiled for this plan (hipRTC, on first use;
                                 the code is cached per process by its
@@ -401,7 +402,8 @@ typedef struct qoa_job_desc {
 } qoa_job_desc;
 /* Plans every job and allocates every instance's state in one device buffer
    (QOA_ERR_OOM when they do not fit). Gates use layer-1 semantics
-   (opts->flags may hold QOA_CZ_REF_MASKS, QOA_BATCH_AUTOTUNE, QOA_FUSE_FACTOR).
+   (opts->flags may hold QOA_CZ_REF_MASKS, QOA_BATCH_AUTOTUNE, QOA_FUSE_FACTOR,
+   QOA_FUSE_MERGE_DIAG).
    QOA_C128 jobs (the default): fused FP64 register passes, tile =
    min(opts->tile_qubits or 12, n), trajectory noise allowed.
    QOA_C64 jobs (all jobs of a batch share one precision): the FP32-fused

@@ -117,6 +117,7 @@ struct qoa_batch {
   uint64_t* frames;  // 2 X-frame slots per instance (noisy trajectories)
   uint32_t* codes;   // noise draws: per instance one word per op of its job's plan
   bool noisy;
+  bool merged;       // noise-free plans with merged diagonal runs (FMA kernels)
   double alg_bytes;  // algorithmic HBM bytes of one run
   hipEvent_t ev0, ev1;
   float last_ms;
@@ -180,7 +181,9 @@ int autotune_tiles(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* ga
     auto it = rep.find(n);
     if (it == rep.end() || jobs[j].n_gates > jobs[it->second].n_gates) rep[n] = j;
   }
-  const uint32_t fl = opts ? (opts->flags & QOA_CZ_REF_MASKS) : 0u;
+  // trials in the arithmetic the batch will run
+  const uint32_t fl =
+      opts ? (opts->flags & (QOA_CZ_REF_MASKS | QOA_FUSE_FACTOR | QOA_FUSE_MERGE_DIAG)) : 0u;
   std::map<uint32_t, uint32_t> best_of;
   for (const auto& kv : rep) {
     const uint32_t n = kv.first, j = kv.second;
@@ -409,6 +412,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
   b->frames = nullptr;
   b->codes = nullptr;
   b->noisy = false;
+  b->merged = false;
   b->ev0 = b->ev1 = nullptr;
   b->last_ms = 0.f;
   b->precision = prec;
@@ -450,7 +454,12 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
     for (uint32_t j = 0; j < n_jobs; ++j) b->job_tile[j] = std::min(b->job_tile[j], jobs[j].n_qubits);
   }
 
-  // plan every job (host threads)
+  // plan every job (host threads); merged diagonal runs only when no job is
+  // noisy (their FMA kernels have no noise code)
+  bool any_noisy = false;
+  for (uint32_t j = 0; j < n_jobs; ++j) any_noisy = any_noisy || jobs[j].p1 > 0.0 || jobs[j].p2 > 0.0;
+  const bool merge = opts && (opts->flags & QOA_FUSE_MERGE_DIAG) && !any_noisy;
+  b->merged = merge;
   std::vector<int> rcs(n_jobs, 0);
   {
     const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
@@ -464,7 +473,8 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
           const bool noisy = jobs[j].p1 > 0.0 || jobs[j].p2 > 0.0;
           PlanOptions po{b->job_tile[j], opts ? opts->low_qubits : 0u,  // noisy jobs too
                          (opts ? (opts->flags & (QOA_CZ_REF_MASKS | QOA_FUSE_FACTOR)) : 0u) |
-                             kPlanForceReg | kPlanSwaps | (noisy ? kPlanNoise : 0u)};
+                             (merge ? QOA_FUSE_MERGE_DIAG : 0u) | kPlanForceReg | kPlanSwaps |
+                             (noisy ? kPlanNoise : 0u)};
           rcs[j] = build_plan(jobs[j].n_qubits, gates + jobs[j].gate_offset, jobs[j].n_gates,
                               mats, po, &b->plans[j]);
         }
@@ -574,6 +584,7 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
       WorkItem w{};
       w.state_off = in.off;
       w.noise = (jd.p1 > 0.0 || jd.p2 > 0.0) ? 1 : 0;
+      w.fma = b->merged ? 1 : 0;
       w.codes_off = codes_base[ii] - (uint64_t)ops_base[in.job];
       w.traj = in.traj;
       w.frame_in = 2 * ii + (r & 1);

@@ -30,7 +30,8 @@ struct WorkItem {
   uint32_t block_begin;  // first block of this item in its round
   uint32_t traj;
   uint32_t frame_in, frame_out;  // slots of the X frame before / after the pass
-  uint8_t k, init, noise, pad0;
+  uint8_t k, init, noise;
+  uint8_t fma;      // the item's plan has merged diagonal runs: FMA kernels
   uint32_t spw[4];  // tile qubit positions as in Geo (whole words: scalar loads)
   uint8_t pad1[20];
 };
@@ -43,7 +44,9 @@ static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
 // VGPRs (three waves per SIMD); the noise-free swap-free kernel, whose blocks
 // have no idle lanes, at most 128 (it takes 108: four waves per SIMD, measured
 // 1% faster than a 96-VGPR five-wave build with two spilled registers).
-template <typename I, bool NOISE, bool SWAPS>
+// F: FMA arithmetic with merged diagonal runs (noise-free batches planned
+// with QOA_FUSE_MERGE_DIAG).
+template <typename I, bool NOISE, bool SWAPS, bool F = false>
 __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(NOISE || SWAPS ? 3 : 4)))
 void k_batch_reg(double2* __restrict__ states,
                                                             const WorkItem* __restrict__ items,
@@ -73,7 +76,7 @@ void k_batch_reg(double2* __restrict__ states,
   nz.on = it.noise;
   nz.codes = codes + (it.codes_off + it.ops_begin);
   nz.frame = (it.noise && !it.init) ? frames[it.frame_in] : 0ULL;
-  const uint64_t fout = reg_tile<I, false, NOISE, true, SWAPS>(states + it.state_off, base, geo, ops + it.ops_begin,
+  const uint64_t fout = reg_tile<I, F, NOISE, true, SWAPS>(states + it.state_off, base, geo, ops + it.ops_begin,
                                     (int)it.nops, pool, lds, act, it.init != 0, nz);
   // every tile of the instance computes the same frame; the first one stores it
   if (NOISE && it.noise && blockIdx.x == it.block_begin && threadIdx.x == 0)
@@ -86,6 +89,7 @@ struct ItemGroup {
   uint32_t threads = 0, kmax = kRegBits;
   uint64_t blocks = 0;
   bool noise = false;
+  bool fma = false;
 };
 
 // Orders items (with nblk[i] tiles each) into groups of equal block size,
@@ -105,6 +109,7 @@ inline void group_items(const std::vector<WorkItem>& in, const std::vector<uint6
       g.blocks += nblk[i];
       g.kmax = std::max<uint32_t>(g.kmax, w.k);
       g.noise = g.noise || w.noise != 0;
+      g.fma = g.fma || w.fma != 0;
       out->push_back(w);
     }
     g.count = (uint32_t)out->size() - g.first;
@@ -115,9 +120,11 @@ inline void group_items(const std::vector<WorkItem>& in, const std::vector<uint6
 // Lets every instantiation take dynamic LDS beyond the default limit (the
 // exchange buffer holds 2^(k - reg_xchg_split(k, true)) amplitudes).
 inline void batch_kernel_attrs() {
-  const void* fns[3] = {(const void*)k_batch_reg<uint32_t, false, false>,
+  const void* fns[5] = {(const void*)k_batch_reg<uint32_t, false, false>,
                         (const void*)k_batch_reg<uint32_t, false, true>,
-                        (const void*)k_batch_reg<uint32_t, true, true>};
+                        (const void*)k_batch_reg<uint32_t, true, true>,
+                        (const void*)k_batch_reg<uint32_t, false, false, true>,
+                        (const void*)k_batch_reg<uint32_t, false, true, true>};
   for (const void* f : fns)
     (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 139264);
 }
@@ -132,7 +139,10 @@ inline void launch_group(hipStream_t st, const ItemGroup& g, double2* states,
                          const WorkItem* d_items, const TileOp* ops, const double* pool,
                          uint64_t* frames, const uint32_t* codes) {
   const size_t lbytes = sizeof(double2) * reg_xchg_elems(g.kmax, true);
+  // (merged plans exist only in noise-free batches)
   auto fn = g.noise ? k_batch_reg<uint32_t, true, true>
+            : g.fma ? (g.kmax <= 10 ? k_batch_reg<uint32_t, false, true, true>
+                                    : k_batch_reg<uint32_t, false, false, true>)
                     : (g.kmax <= 10 ? k_batch_reg<uint32_t, false, true>
                                     : k_batch_reg<uint32_t, false, false>);
   hipLaunchKernelGGL(fn, dim3((unsigned)g.blocks), dim3(g.threads), lbytes, st, states,

@@ -40,16 +40,19 @@ def test_ansatz_batch_matches_oracle(tile):
             assert np.array_equal(b.download(j), first[j])
 
 
+@pytest.mark.parametrize("merge", [False, True])
 @pytest.mark.parametrize("tile", [8, 10, 12])
-def test_factored_ansatz_batch_within_tolerance(tile):
+def test_factored_ansatz_batch_within_tolerance(tile, merge):
     """QOA_FUSE_FACTOR for batches: RY as [[1, -t], [t, 1]] with the cosine
     applied once per pass, RZ as a phase on half the registers; every job
-    within 1e-12 of the oracle (original order)."""
+    within 1e-12 of the oracle (original order); with QOA_FUSE_MERGE_DIAG
+    too (runs of RZ phases and CZ signs merged, FMA kernels)."""
     jobs = []
     for j in range(24):
         n, depth, g = W.ansatz_job(700 + j, n_min=4, n_max=15, d_min=2, d_max=12)
         jobs.append((n, g))
-    with Batch(jobs, tile_qubits=tile, flags=_native.QOA_FUSE_FACTOR) as b:
+    flags = _native.QOA_FUSE_FACTOR | (_native.QOA_FUSE_MERGE_DIAG if merge else 0)
+    with Batch(jobs, tile_qubits=tile, flags=flags) as b:
         b.run()
         b.synchronize()
         for j, (n, g) in enumerate(jobs):
