@@ -12,7 +12,15 @@
 // runs per load. For a (compressed) target below 6 the partner sits in the
 // same wavefront: each lane loads one amplitude of a fully contiguous run and
 // fetches its partner with a cross-lane shuffle instead.
+//
+// Every sweep reads and writes each amplitude it visits once, so its loads
+// and stores carry the nontemporal hint (ld_stream / st_stream; measured on
+// the 28-qubit layered circuit: dense sweeps 5.60 -> 6.04 TB/s, diagonal
+// 5.25 -> 5.38, permutations 5.25 -> 5.43).
 #include "qoa_internal.hpp"
+#include "tile_device.hpp"
+using qoa::tile::ld_stream;
+using qoa::tile::st_stream;
 
 #pragma clang fp contract(off)
 
@@ -95,8 +103,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(double2* __restrict__ a, I npai
       const I p = base + (I)(j * kBlock);
       i0[j] = deposit(p, ins) | cmask;
       if (p < npairs) {
-        x[j] = a[i0[j]];
-        y[j] = a[i0[j] | tbit];
+        x[j] = ld_stream(a + i0[j]);
+        y[j] = ld_stream(a + (i0[j] | tbit));
       }
     }
 #pragma unroll
@@ -105,8 +113,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(double2* __restrict__ a, I npai
       if (p < npairs) {
         const double2 o0 = op.out0(x[j], y[j]);
         const double2 o1 = op.out1(x[j], y[j]);
-        a[i0[j]] = o0;
-        a[i0[j] | tbit] = o1;
+        st_stream(a + i0[j], o0);
+        st_stream(a + (i0[j] | tbit), o1);
       }
     }
   }
@@ -127,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void k_shfl(double2* __restrict__ a, I nele
       const I p = base + (I)(j * kBlock);
       e[j] = deposit(p, ins) | cmask;
       v[j] = make_double2(0.0, 0.0);
-      if (p < nelem) v[j] = a[e[j]];
+      if (p < nelem) v[j] = ld_stream(a + e[j]);
     }
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_shfl(double2* __restrict__ a, I nele
       o.y = __shfl_xor(v[j].y, lm);
       const double2 r = hi ? op.out1(o, v[j]) : op.out0(v[j], o);
       const I p = base + (I)(j * kBlock);
-      if (p < nelem) a[e[j]] = r;
+      if (p < nelem) st_stream(a + e[j], r);
     }
   }
 }
@@ -152,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(double2* __restrict__ a, I coun
     for (int j = 0; j < kItems; ++j) {
       const I p = base + (I)(j * kBlock);
       e[j] = deposit(p, ins) | mask;
-      if (p < count) v[j] = a[e[j]];
+      if (p < count) v[j] = ld_stream(a + e[j]);
     }
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
@@ -161,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(double2* __restrict__ a, I coun
         double2 r;
         if (NEG) r = make_double2(v[j].x * -1.0, v[j].y * -1.0);  // Complex::scale(-1.0)
         else r = cmul(v[j], f);
-        a[e[j]] = r;
+        st_stream(a + e[j], r);
       }
     }
   }
@@ -176,12 +184,12 @@ __global__ __launch_bounds__(kBlock) void k_rz(double2* __restrict__ a, I len, I
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
       const I i = base + (I)(j * kBlock);
-      if (i < len) v[j] = a[i];
+      if (i < len) v[j] = ld_stream(a + i);
     }
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
       const I i = base + (I)(j * kBlock);
-      if (i < len) a[i] = (i & bit) ? cmul(v[j], m11) : cmul(v[j], m00);
+      if (i < len) st_stream(a + i, (i & bit) ? cmul(v[j], m11) : cmul(v[j], m00));
     }
   }
 }
@@ -197,9 +205,9 @@ __global__ __launch_bounds__(kBlock) void k_swap2(double2* __restrict__ a, I cou
       const I p = base + (I)(j * kBlock);
       if (p < count) {
         const I b = deposit(p, ins) | cmask;
-        const double2 u = a[b | m2], w = a[b | m1];
-        a[b | m2] = w;
-        a[b | m1] = u;
+        const double2 u = ld_stream(a + (b | m2)), w = ld_stream(a + (b | m1));
+        st_stream(a + (b | m2), w);
+        st_stream(a + (b | m1), u);
       }
     }
   }
