@@ -247,7 +247,10 @@ def kernel_table(sv) -> dict:
     return kinds
 
 
-def run_circuit(sv, gates, method: str, steps: int, warmup: int, tag: str):
+def run_circuit(sv, gates, method: str, steps: int, warmup: int, tag: str,
+                timing: bool = True):
+    """Times `steps` back-to-back runs (after `warmup` waited-for ones); with
+    timing, HIP events around every kernel feed the per-kind table."""
     def step():
         if method == "per-gate":
             sv.run_gates(gates)
@@ -272,15 +275,16 @@ def run_circuit(sv, gates, method: str, steps: int, warmup: int, tag: str):
         step()
         sv.synchronize()
         log(f"{tag}: warmup {w + 1}/{warmup} done")
-    sv.timing_enable(True)
+    sv.timing_enable(timing)
     sv.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
         step()
-        log(f"{tag}: step {k + 1}/{steps} enqueued")
+        if timing:
+            log(f"{tag}: step {k + 1}/{steps} enqueued")
     sv.synchronize()
     elapsed = time.perf_counter() - t0
-    kinds = kernel_table(sv)
+    kinds = kernel_table(sv) if timing else {}
     sv.timing_enable(False)
     return elapsed, kinds
 
@@ -296,13 +300,14 @@ def extra_c1(args) -> dict:
                    "fused-relabel: the headline's mode (specialised, factored, relabelled; "
                    "compiled in the warm-up calls), compared with the per-gate run"}
     with StateVector(n) as sv, StateVector(n) as ref:
-        el, _ = run_circuit(ref, g, "per-gate", 20, 2, "c1 per-gate")
+        # no per-kernel events here: at 16 qubits they would cost as much as the kernels
+        el, _ = run_circuit(ref, g, "per-gate", 20, 2, "c1 per-gate", timing=False)
         out["per-gate_us_per_circuit"] = el / 20 * 1e6
         out["per-gate_circuits_per_s"] = 20 / el
         ref.apply_reset_all()
         ref.run_gates(g)
         for method in ("fused", "fused-relabel"):
-            el, _ = run_circuit(sv, g, method, 20, 2, f"c1 {method}")
+            el, _ = run_circuit(sv, g, method, 20, 2, f"c1 {method}", timing=False)
             out[f"{method}_us_per_circuit"] = el / 20 * 1e6
             out[f"{method}_circuits_per_s"] = 20 / el
             sv.apply_reset_all()
