@@ -594,6 +594,11 @@ int qoa_batch_create(const qoa_job_desc* jobs, uint32_t n_jobs, const qoa_gate* 
       w.k = (uint8_t)P.k;
       w.init = r == 0 ? 1 : 0;
       std::memcpy(w.spw, pp.spos, 16);
+      std::memcpy(w.stw, pp.spos_st, 16);
+      {
+        uint32_t spw[4];
+        pass_geo_words(pp, spw, w.stw, w.srt);
+      }
       items.push_back(w);
       nblk.push_back(1ULL << (in.n - P.k));
       b->alg_bytes += (r == 0 ? 16.0 : 32.0) * (double)(1ULL << in.n);

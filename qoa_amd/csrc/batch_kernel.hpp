@@ -32,10 +32,12 @@ struct WorkItem {
   uint32_t frame_in, frame_out;  // slots of the X frame before / after the pass
   uint8_t k, init, noise;
   uint8_t fma;      // the item's plan has merged diagonal runs: FMA kernels
-  uint32_t spw[4];  // tile qubit positions as in Geo (whole words: scalar loads)
-  uint8_t pad1[20];
+  uint32_t spw[4];  // tile qubit load positions as in Geo (whole words: scalar loads)
+  uint32_t stw[4];  // and store positions (spw unless the plan relabels qubits)
+  uint32_t srt[4];  // the positions ascending (the tile base)
+  uint8_t pad1[4];
 };
-static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
+static_assert(sizeof(WorkItem) == 96, "WorkItem layout");
 
 // NOISE = false: no item of the launch has trajectory noise. SWAPS: the
 // items' plans may hold swap-list layout ops (tiles of up to 10 qubits,
@@ -63,10 +65,20 @@ void k_batch_reg(double2* __restrict__ states,
     else hi = mid - 1;
   }
   const WorkItem& it = items[lo];
-  GeoSame geo;
+  // Batch plans do not relabel (stw = spw, srt = spw). The noisy kernel reads
+  // the positions from the item (measured: 161 -> 152 VGPRs, scratch 44 -> 24
+  // bytes per lane, serving 423 -> 445 jobs/s); the noise-free ones keep
+  // their five words in registers (pointer reads measured 6 % slower there).
+  std::conditional_t<NOISE, GeoItem, GeoSame> geo;
   geo.k = it.k;
+  if constexpr (NOISE) {
+    geo.spw = it.spw;
+    geo.stw = it.stw;
+    geo.srt = it.srt;
+  } else {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) geo.spw[j] = it.spw[j];
+    for (int j = 0; j < 4; ++j) geo.spw[j] = it.spw[j];
+  }
   const I base = tile_base((I)(blockIdx.x - it.block_begin), geo);
   // Groups of 11+-qubit tiles (the only ones the swap-free kernel runs) have
   // exactly 2^(k-4) threads per block; one-wave blocks of smaller tiles have

@@ -992,8 +992,10 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
   for (uint32_t i = 0; i < ops.size(); ++i) rem[i] = i;
   const bool reg_ok = (k >= 8 || (opt.flags & kPlanForceReg)) && k >= (uint32_t)kRegBits &&
                       !(opt.flags & QOA_FUSE_LDS_ONLY);
+  // (not with swap-list layouts, nor with noise: trajectory errors name
+  // their qubits by position)
   const bool relabel = (opt.flags & QOA_FUSE_RELABEL) && !(opt.flags & kPlanNoise) &&
-                       !(opt.flags & kPlanSwaps) && reg_ok && k < n && m > 0 &&
+                       !use_swaps(k, (opt.flags & kPlanSwaps) != 0) && reg_ok && k < n && m > 0 &&
                        k >= (uint32_t)kRegBits + 2 * m;
   // logical qubit q sits at physical position pos[q]; inv is the inverse
   uint8_t pos[64], inv[64];

@@ -333,6 +333,11 @@ int launch_step(qoa_server* s) {
       w.k = (uint8_t)j->plan.k;
       w.init = r == 0 ? 1 : 0;
       std::memcpy(w.spw, pp.spos, 16);
+      std::memcpy(w.stw, pp.spos_st, 16);
+      {
+        uint32_t spw[4];
+        pass_geo_words(pp, spw, w.stw, w.srt);
+      }
       built.push_back(w);
       nblk.push_back(len >> j->plan.k);
       ++nitems;

@@ -55,6 +55,25 @@ __device__ __forceinline__ uint32_t spos_at(const GeoSame& g, uint32_t j) { retu
 __device__ __forceinline__ uint32_t stpos_at(const GeoSame& g, uint32_t j) { return byte_at(g.spw, j); }
 __device__ __forceinline__ uint32_t srt_at(const GeoSame& g, uint32_t j) { return byte_at(g.spw, j); }
 
+// The geometry of batch work items, read where it lies in the item (global
+// memory, uniform: scalar loads): load positions, store positions (equal
+// unless the item's plan relabels qubits) and the positions ascending (the
+// tile base). A copy in a local struct would sit in scratch memory, since the
+// noise code indexes it with a loop variable.
+struct GeoItem {
+  int k;
+  const uint32_t* spw;
+  const uint32_t* stw;
+  const uint32_t* srt;
+};
+__device__ __forceinline__ uint32_t word_byte(const uint32_t* w, uint32_t j) {
+  return (w[j >> 2] >> (8 * (j & 3u))) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t spos_at(const GeoItem& g, uint32_t j) { return word_byte(g.spw, j); }
+__device__ __forceinline__ uint32_t stpos_at(const GeoItem& g, uint32_t j) { return word_byte(g.stw, j); }
+__device__ __forceinline__ uint32_t srt_at(const GeoItem& g, uint32_t j) { return word_byte(g.srt, j); }
+
+
 // The tile base of tile index t: t's bits spread over the positions outside
 // the tile (a zero bit inserted at each tile position, ascending).
 template <typename I, typename G>
