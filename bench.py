@@ -863,6 +863,12 @@ def roofline(dom, args, copy=None) -> dict:
     if copy:
         out["copy_ceiling"] = copy
         out["frac_of_copy"] = k["gbps"] / copy["gbps"]
+        # whole-step view: passes per step and the step time if every pass
+        # moved its bytes at the copy sweep's rate (fewer, fuller passes lower
+        # the per-launch fraction while the step gets faster)
+        per_step = k["launches"] / max(args.steps, 1)
+        out["launches_per_step"] = per_step
+        out["ms_per_step_at_copy_rate"] = per_step * per_launch / (copy["gbps"] * 1e9) * 1e3
     default = (args.qubits, args.depth, args.seed) == (28, 40, 20241203)
     key = {"fused": "fused_pass/28q-d40-ladders",
            "fused-merge": "fused_pass_merge/28q-d40-ladders",
