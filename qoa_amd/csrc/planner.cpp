@@ -914,7 +914,7 @@ namespace {
 // The pass's tile qubits: from `seed`, greedily the qubit (or, when no single
 // one helps, the pair) that lets the most ops of `rem` run, up to k qubits.
 uint64_t choose_set(const std::vector<FOp>& ops, const std::vector<uint32_t>& rem, uint64_t seed,
-                    uint32_t n, uint32_t k, uint64_t allq) {
+                    uint32_t n, uint32_t k, uint64_t allq, uint64_t prefer = 0) {
   uint64_t S = seed;
   int cur = simulate(ops, rem, S, allq);
   while ((uint32_t)__builtin_popcountll(S) < k) {
@@ -922,7 +922,9 @@ uint64_t choose_set(const std::vector<FOp>& ops, const std::vector<uint32_t>& re
     for (uint32_t q = 0; q < n; ++q) {
       if (S & bit(q)) continue;
       const int sc = simulate(ops, rem, S | bit(q), allq);
-      if (sc > bests) {
+      // ties go to the preferred qubits (relabelled plans: qubits away from
+      // their own position, so later passes can put them back)
+      if (sc > bests || (sc == bests && (prefer & bit(q)) && !(prefer & bit((uint32_t)best)))) {
         bests = sc;
         best = (int)q;
       }
@@ -1101,7 +1103,12 @@ int build_plan(uint32_t n, const qoa_gate* g, uint32_t ng, const double* mats,
   };
 
   while (!rem.empty()) {
-    uint64_t S = k >= n ? allq : choose_set(ops, rem, low_now() | ops[rem[0]].active, n, k, allq);
+    uint64_t mis = 0;
+    if (relabel)
+      for (uint32_t q = 0; q < n; ++q)
+        if (pos[q] != q) mis |= bit(q);
+    uint64_t S =
+        k >= n ? allq : choose_set(ops, rem, low_now() | ops[rem[0]].active, n, k, allq, mis);
     if (relabel) {
       // The last pass: when the qubits its ops mix, the low ones and every
       // qubit away from its own position fit in one tile, take that tile, so
